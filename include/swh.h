@@ -1,0 +1,113 @@
+/*
+ * swh.h -- C ABI of the MI355X Smith-Waterman search library (libswh.so).
+ *
+ * Plain pointers and sizes only.  Every call returns an int status
+ * (SWH_OK = 0, negative on failure) and leaves a message for swh_last_error()
+ * on the calling thread; the library never exits the process.  Calls on one
+ * context must be serialised by the caller; distinct contexts are independent.
+ *
+ * Conventions
+ *   residue codes   0..23 (A0 B1 C2 D3 E4 F5 G6 H7 I8 K9 L10 M11 N12 P13 Q14
+ *                   R15 S16 T17 V18 W19 X20 Y21 Z22, 23 = pad), as produced by
+ *                   the reference's translation table (src/DataPool.cpp:33-47).
+ *                   Database codes above 23 are treated as 23, as the
+ *                   reference's packDB does (src/Worker.cpp:150-154); query
+ *                   codes above 23 are rejected.
+ *   matrix          32x32 int32, row = query code, column = database code
+ *                   (src/Param.h:110-115).  Entries must lie in [-128, 127].
+ *   gaps            gap_open, gap_extend as the reference's -o / -e flags; the
+ *                   first gap residue costs gap_open + gap_extend
+ *                   (src/Param.h:109).
+ *   scores          exact int32 local-alignment scores, one per database
+ *                   sequence, in the caller's sequence order.
+ */
+#ifndef SWH_H_
+#define SWH_H_
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SWH_OK 0
+#define SWH_ERR_ARG (-1)
+#define SWH_ERR_HIP (-2)
+#define SWH_ERR_NOMEM (-3)
+#define SWH_ERR_STATE (-4)
+#define SWH_ERR_CAPACITY (-5)
+
+typedef struct swh_ctx swh_ctx;
+typedef struct swh_db swh_db;
+
+/* Timing and counters of the last search on a context. */
+typedef struct swh_stats {
+    double kernel_ms;         /* int16 scan kernels (HIP events on the search stream) */
+    double rescore_ms;        /* int32 rescore kernel */
+    double total_ms;          /* whole call, host wall clock */
+    uint64_t cells;           /* sum over sequences of len(db seq) * len(query) */
+    uint64_t padded_cells;    /* cells actually computed, padding included */
+    uint64_t n_overflow;      /* sequences rescored in int32 */
+    uint64_t n_launches;      /* kernel launches issued by the call */
+} swh_stats;
+
+/* Per-call tuning.  Zero-initialise for defaults. */
+typedef struct swh_search_opts {
+    int strip_width;          /* query columns held in registers per lane:
+                                 0 = auto, else 16, 24 or 32 */
+    int overflow_threshold;   /* 0 = exact default (65535 - max(matrix));
+                                 a lower value forces more int32 rescoring */
+    int group_min_len;        /* database length from which whole wavefronts
+                                 cooperate on one sequence pair; 0 = auto,
+                                 <0 = never */
+    int use_graph;            /* batched calls: capture launches in a hipGraph */
+} swh_search_opts;
+
+const char* swh_last_error(void);
+int swh_version(void);
+int swh_device_count(int* count);
+
+/* Context: device, scoring scheme, search stream and scratch.
+ * Replaces CUDAWorker's constructor/alloc (src/CUDAWorker.cu:29-98) and the
+ * Param scoring fields it reads (src/Param.h:59-117). */
+int swh_ctx_create(int device, const int32_t* matrix, int gap_open, int gap_extend,
+                   swh_ctx** out);
+int swh_ctx_destroy(swh_ctx* ctx);
+int swh_ctx_set_scoring(swh_ctx* ctx, const int32_t* matrix, int gap_open, int gap_extend);
+
+/* Database: sort by length, interleave into the HBM layout and upload once.
+ * Replaces CUDAWorker::packDB + asyncCopy (src/CUDAWorker.cu:144-223), which
+ * re-packed and re-copied every 256 MB chunk.  residues holds every sequence
+ * back to back; sequence i is residues[offsets[i] .. offsets[i]+lengths[i]). */
+int swh_db_create(swh_ctx* ctx, const uint8_t* residues, const uint64_t* offsets,
+                  const uint32_t* lengths, uint64_t nseq, swh_db** out);
+int swh_db_destroy(swh_db* db);
+/* nseq, total residues, device bytes held by the packed layout. */
+int swh_db_info(const swh_db* db, uint64_t* nseq, uint64_t* residues, uint64_t* device_bytes);
+
+/* One query, one search: int16 scan + on-device int32 rescore; scores[i] for
+ * every database sequence in caller order.
+ * Replaces kernelLaunch + bindQueryPrf (src/CUDAKernel.h:9-22,
+ * src/CUDAKernel.cu:194-226) and the score copy-back of CUDAWorker::search
+ * (src/CUDAWorker.cu:224-246). */
+int swh_search(swh_ctx* ctx, swh_db* db, const uint8_t* query, int qlen,
+               const swh_search_opts* opts, int32_t* scores, swh_stats* stats);
+
+/* Many queries in one persistent launch with a device work queue; scores is
+ * nq rows of nseq int32 in query order.  No reference counterpart: the
+ * reference searches one query per process (src/main.cpp:7-16). */
+int swh_search_batch(swh_ctx* ctx, swh_db* db, const uint8_t* const* queries,
+                     const int* qlens, int nq, const swh_search_opts* opts,
+                     int32_t* scores, swh_stats* stats);
+
+/* Top-k of a score vector: (score desc, index asc), exactly k entries, padded
+ * with (0, 0) -- the ResultList order (src/ResultList.h:37-79) made
+ * deterministic for ties. */
+int swh_topk(const int32_t* scores, uint64_t n, int k, int32_t* out_score,
+             int64_t* out_index);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

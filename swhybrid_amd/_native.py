@@ -1,0 +1,98 @@
+"""ctypes binding of libswh.so (the C ABI declared in include/swh.h).
+
+The shared object is built in-tree (swhybrid_amd/libswh.so).  Loading it
+never falls back to anything: a missing or unloadable library raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libswh.so")
+HOST_LIB_PATH = os.path.join(HERE, "libswhybrid_host.so")
+
+SWH_OK = 0
+
+# Every symbol include/swh.h declares (tests check the library exports them).
+EXPORTS = (
+    "swh_last_error", "swh_version", "swh_device_count", "swh_ctx_create",
+    "swh_ctx_destroy", "swh_ctx_set_scoring", "swh_db_create", "swh_db_destroy",
+    "swh_db_info", "swh_search", "swh_search_batch", "swh_topk",
+)
+
+
+class SwhStats(C.Structure):
+    _fields_ = [
+        ("kernel_ms", C.c_double),
+        ("rescore_ms", C.c_double),
+        ("total_ms", C.c_double),
+        ("cells", C.c_uint64),
+        ("padded_cells", C.c_uint64),
+        ("n_overflow", C.c_uint64),
+        ("n_launches", C.c_uint64),
+    ]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class SwhSearchOpts(C.Structure):
+    _fields_ = [
+        ("strip_width", C.c_int),
+        ("overflow_threshold", C.c_int),
+        ("group_min_len", C.c_int),
+        ("use_graph", C.c_int),
+    ]
+
+
+class SwhError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    """Load libswh.so (raises if it is missing: there is no fallback path)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise SwhError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                       "or `make -C swhybrid_amd/csrc`")
+    L = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    L.swh_last_error.restype = C.c_char_p
+    L.swh_last_error.argtypes = []
+    L.swh_version.restype = C.c_int
+    L.swh_device_count.argtypes = [C.POINTER(C.c_int)]
+    L.swh_ctx_create.argtypes = [C.c_int, vp, C.c_int, C.c_int, C.POINTER(vp)]
+    L.swh_ctx_destroy.argtypes = [vp]
+    L.swh_ctx_set_scoring.argtypes = [vp, vp, C.c_int, C.c_int]
+    L.swh_db_create.argtypes = [vp, vp, vp, vp, C.c_uint64, C.POINTER(vp)]
+    L.swh_db_destroy.argtypes = [vp]
+    L.swh_db_info.argtypes = [vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
+                              C.POINTER(C.c_uint64)]
+    L.swh_search.argtypes = [vp, vp, vp, C.c_int, C.POINTER(SwhSearchOpts), vp,
+                             C.POINTER(SwhStats)]
+    L.swh_search_batch.argtypes = [vp, vp, vp, vp, C.c_int, C.POINTER(SwhSearchOpts), vp,
+                                   C.POINTER(SwhStats)]
+    L.swh_topk.argtypes = [vp, C.c_uint64, C.c_int, vp, vp]
+    for name in EXPORTS:
+        if name != "swh_last_error" and name != "swh_version":
+            getattr(L, name).restype = C.c_int
+    _lib = L
+    return L
+
+
+def check(rc: int):
+    if rc != SWH_OK:
+        msg = lib().swh_last_error()
+        raise SwhError(f"libswh error {rc}: {msg.decode() if msg else ''}")
+
+
+def ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)

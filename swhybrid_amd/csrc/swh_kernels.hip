@@ -1,0 +1,376 @@
+// swh_kernels.hip -- gfx950 kernels of the Smith-Waterman database search.
+//
+// Inter-sequence affine-gap local alignment (the recurrence of the reference's
+// ONE_CELL_UPDATE, src/AVXKernel.cpp:196-218):
+//   H(i,j) = max(0, H(i-1,j-1) + s(q_j, d_i), E(i,j), F(i,j))
+//   E(i,j) = max(E(i-1,j) - ext, H(i-1,j) - (open+ext))      (gap along the db)
+//   F(i,j) = max(F(i,j-1) - ext, H(i,j-1) - (open+ext))      (gap along the query)
+//   score  = max H
+// Database sequences run along the rows i, the query along the columns j.
+//
+// Scan kernel (int16): every lane carries TWO database sequences, one in each
+// 16-bit half of a VGPR, and all arithmetic is packed and saturating
+// (v_pk_add_i16 clamp / v_pk_sub_i16 clamp / v_pk_max_i16).  Values are biased
+// so that 0x8000 encodes score 0: the saturating floor at -32768 is the
+// local-alignment zero floor, exactly as the reference's 0x80 / 0x8000 bias
+// (src/AVXKernel.cpp:443, :903).  A lane keeps a strip of W query columns in
+// registers (H, E and H-(open+ext) per column) and walks down its sequences'
+// rows; the H/F values on the strip's right edge go to a per-wave border array
+// in HBM and feed the next strip.  The query profile (24 x qlen int16) is
+// staged in LDS per 1344-column segment; each row reads the profile rows of the
+// lane's two residues with 16-byte LDS loads and interleaves them with one
+// v_perm_b32 per column.
+//
+// A sequence whose int16 score reaches the overflow threshold (default
+// 65535 - max(matrix), below which no add can have saturated) is appended to
+// an overflow list and rescored exactly by the int32 kernel, on the device.
+#include "swh_internal.h"
+
+namespace swh {
+
+typedef short s2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ s2 adds(s2 a, s2 b) { return __builtin_elementwise_add_sat(a, b); }
+__device__ __forceinline__ s2 subs(s2 a, s2 b) { return __builtin_elementwise_sub_sat(a, b); }
+__device__ __forceinline__ s2 vmax(s2 a, s2 b) { return __builtin_elementwise_max(a, b); }
+__device__ __forceinline__ int adds(int a, int b) { return __builtin_elementwise_add_sat(a, b); }
+__device__ __forceinline__ int subs(int a, int b) { return __builtin_elementwise_sub_sat(a, b); }
+__device__ __forceinline__ int vmax(int a, int b) { return a > b ? a : b; }
+__device__ __forceinline__ uint32_t bits(s2 v) { return __builtin_bit_cast(uint32_t, v); }
+__device__ __forceinline__ s2 as_s2(uint32_t v) { return __builtin_bit_cast(s2, v); }
+
+constexpr uint32_t kZero16x2 = 0x80008000u;   // biased zero in both halves
+constexpr int kZero32 = INT32_MIN;             // biased zero, int32 rescore
+
+// Copy profile rows 0..23, columns [seg0, seg0+segw) into LDS rows of stride ps.
+__device__ __forceinline__ void stage_profile(short* lds, const int16_t* __restrict__ prof,
+                                              int qpad, int ps, int seg0, int segw) {
+    const int per_row = segw >> 3;
+    const int total = kProfRows * per_row;
+    for (int t = threadIdx.x; t < total; t += blockDim.x) {
+        const int r = t / per_row;
+        const int c8 = t - r * per_row;
+        const uint4 v = *reinterpret_cast<const uint4*>(prof + (size_t)r * qpad + seg0 + c8 * 8);
+        *reinterpret_cast<uint4*>(lds + r * ps + c8 * 8) = v;
+    }
+}
+
+// One row of a strip: W cells, packed pairs.  P[c] = (s(q_c, a), s(q_c, b)).
+// h/e/hq hold H(i-1,c), E(i-1,c), H(i-1,c)-(open+ext) on entry and row i on
+// exit; M[c] is the running maximum of column c (independent chains, reduced
+// once per strip, so no serial max chain forms across the row).
+template <int W, typename T>
+__device__ __forceinline__ void row_cells(T (&h)[W], T (&e)[W], T (&hq)[W], T (&M)[W],
+                                          const T (&P)[W], T diag0, T hl, T fl, T Q, T R,
+                                          T& fout) {
+    T X = adds(diag0, P[0]);      // H(i-1, c0-1) + s
+    T f = fl;                      // F(i, c0-1)
+    T hqp = subs(hl, Q);           // H(i, c0-1) - (open+ext)
+#pragma unroll
+    for (int c = 0; c < W; ++c) {
+        f = vmax(subs(f, R), hqp);                 // F(i, c)
+        e[c] = vmax(subs(e[c], R), hq[c]);         // E(i, c)
+        T Xn = X;
+        if (c + 1 < W) Xn = adds(h[c], P[c + 1]);  // diagonal term of column c+1
+        T hn = vmax(vmax(X, e[c]), f);             // H(i, c); zero floor by saturation
+        M[c] = vmax(M[c], hn);
+        hq[c] = subs(hn, Q);
+        h[c] = hn;
+        hqp = hq[c];
+        X = Xn;
+    }
+    fout = f;
+}
+
+template <int W, typename T>
+__device__ __forceinline__ T reduce_max(const T (&M)[W]) {
+    T m = M[0];
+#pragma unroll
+    for (int c = 1; c < W; ++c) m = vmax(m, M[c]);
+    return m;
+}
+
+template <int W>
+__device__ __forceinline__ void pair_profile(s2 (&P)[W], const short* lds, int ps, int a, int b,
+                                             int jj0) {
+    const uint4* pa = reinterpret_cast<const uint4*>(lds + __umul24(a, ps) + jj0);
+    const uint4* pb = reinterpret_cast<const uint4*>(lds + __umul24(b, ps) + jj0);
+#pragma unroll
+    for (int k = 0; k < W / 8; ++k) {
+        const uint4 va = pa[k];
+        const uint4 vb = pb[k];
+        P[8 * k + 0] = as_s2(__builtin_amdgcn_perm(vb.x, va.x, 0x05040100u));
+        P[8 * k + 1] = as_s2(__builtin_amdgcn_perm(vb.x, va.x, 0x07060302u));
+        P[8 * k + 2] = as_s2(__builtin_amdgcn_perm(vb.y, va.y, 0x05040100u));
+        P[8 * k + 3] = as_s2(__builtin_amdgcn_perm(vb.y, va.y, 0x07060302u));
+        P[8 * k + 4] = as_s2(__builtin_amdgcn_perm(vb.z, va.z, 0x05040100u));
+        P[8 * k + 5] = as_s2(__builtin_amdgcn_perm(vb.z, va.z, 0x07060302u));
+        P[8 * k + 6] = as_s2(__builtin_amdgcn_perm(vb.w, va.w, 0x05040100u));
+        P[8 * k + 7] = as_s2(__builtin_amdgcn_perm(vb.w, va.w, 0x07060302u));
+    }
+}
+
+// One strip of W columns over every row of a lane-mode block.
+template <int W>
+__device__ __forceinline__ void lane_strip(const uint2* __restrict__ wds, int quads,
+                                           uint2* __restrict__ bord, const short* lds, int ps,
+                                           int jj0, bool has_in, bool has_out, s2 Q, s2 R,
+                                           s2& S, int lane) {
+    const s2 Z = as_s2(kZero16x2);
+    s2 h[W], e[W], hq[W], M[W];
+#pragma unroll
+    for (int c = 0; c < W; ++c) {
+        h[c] = Z;
+        e[c] = Z;
+        hq[c] = Z;
+        M[c] = Z;
+    }
+    s2 diag0 = Z;
+    const uint2 zz = make_uint2(kZero16x2, kZero16x2);
+    uint2 wn = wds[lane];
+    uint2 bn[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) bn[r] = has_in ? bord[r * kWave + lane] : zz;
+    for (int k = 0; k < quads; ++k) {
+        const uint2 w = wn;
+        uint2 bc[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) bc[r] = bn[r];
+        if (k + 1 < quads) {
+            wn = wds[(size_t)(k + 1) * kWave + lane];
+            if (has_in) {
+                const uint2* bp = bord + (size_t)(k + 1) * 4 * kWave + lane;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) bn[r] = bp[r * kWave];
+            }
+        }
+        uint2* bo = bord + (size_t)k * 4 * kWave + lane;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int a = (w.x >> (8 * r)) & 0xff;
+            const int b = (w.y >> (8 * r)) & 0xff;
+            s2 P[W];
+            pair_profile<W>(P, lds, ps, a, b, jj0);
+            const s2 hl = as_s2(bc[r].x);
+            const s2 fl = as_s2(bc[r].y);
+            s2 fout;
+            row_cells<W, s2>(h, e, hq, M, P, diag0, hl, fl, Q, R, fout);
+            if (has_out) bo[r * kWave] = make_uint2(bits(h[W - 1]), bits(fout));
+            diag0 = hl;
+        }
+    }
+    S = vmax(S, reduce_max<W, s2>(M));
+}
+
+__device__ __forceinline__ void emit(const ScanArgs& A, uint32_t s, int sc) {
+    const uint32_t idx = A.perm[s];
+    if (idx == kNoSeq) return;
+    if (sc >= A.ovf_threshold) {
+        const uint32_t slot = atomicAdd(A.ovf_count, 1u);
+        A.ovf_list[slot] = s;
+        A.scores[idx] = -1;
+    } else {
+        A.scores[idx] = sc;
+    }
+}
+
+template <int W>
+__global__ __launch_bounds__(256) void k_lane_scan(ScanArgs A) {
+    extern __shared__ __align__(16) short lds[];
+    __shared__ int s_item;
+    const int wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    uint2* bord = A.border + (size_t)(blockIdx.x * kWavesPerWG + wave) * A.border_stride;
+    const s2 Q = {(short)A.gap_oe, (short)A.gap_oe};
+    const s2 R = {(short)A.gap_e, (short)A.gap_e};
+    const int nseg = (A.qpad + kSegCols - 1) / kSegCols;
+    int loaded = -1;
+    for (;;) {
+        __syncthreads();
+        if (threadIdx.x == 0) s_item = (int)atomicAdd(A.work_counter, 1u);
+        __syncthreads();
+        const int item = s_item;
+        if (item >= A.n_items) break;
+        const int blk = A.blk_begin + item * kWavesPerWG + wave;
+        const bool active = blk < A.blk_end;
+        BlockDesc bd = {0, 0, 0};
+        if (active) bd = A.blocks[blk];
+        const uint2* wds = reinterpret_cast<const uint2*>(A.words + bd.word_off);
+        const int quads = (int)(bd.rows >> 2);
+        s2 S = as_s2(kZero16x2);
+        for (int seg = 0; seg < nseg; ++seg) {
+            const int seg0 = seg * kSegCols;
+            const int segw = min(kSegCols, A.qpad - seg0);
+            if (seg != loaded) {
+                __syncthreads();
+                stage_profile(lds, A.prof, A.qpad, A.ps, seg0, segw);
+                __syncthreads();
+                loaded = seg;
+            }
+            if (active) {
+                for (int c0 = seg0; c0 < seg0 + segw; c0 += W)
+                    lane_strip<W>(wds, quads, bord, lds, A.ps, c0 - seg0, c0 > 0, c0 + W < A.qpad,
+                                  Q, R, S, lane);
+            }
+        }
+        if (active) {
+            emit(A, bd.first + lane, (int)S.x + 32768);
+            emit(A, bd.first + kWave + lane, (int)S.y + 32768);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// int32 rescore: one overflowed sequence per lane, same strip scheme with
+// 32-bit saturating arithmetic biased by INT32_MIN.
+template <int W>
+__device__ __forceinline__ void single_profile(int (&P)[W], const short* lds, int ps, int a,
+                                               int jj0) {
+    const short* pa = lds + __umul24(a, ps) + jj0;
+#pragma unroll
+    for (int c = 0; c < W; ++c) P[c] = pa[c];
+}
+
+__global__ __launch_bounds__(256) void k_rescore(RescoreArgs A) {
+    constexpr int W = 8;   // divides every padded query length (all are multiples of 8)
+    extern __shared__ __align__(16) short lds[];
+    __shared__ int s_item;
+    const int wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    uint2* bord = A.border + (size_t)(blockIdx.x * kWavesPerWG + wave) * A.border_stride;
+    const int n = (int)*A.ovf_count;
+    const int n_items = (n + 255) >> 8;
+    const int nseg = (A.qpad + kSegCols - 1) / kSegCols;
+    int loaded = -1;
+    for (;;) {
+        __syncthreads();
+        if (threadIdx.x == 0) s_item = (int)atomicAdd(A.work_counter, 1u);
+        __syncthreads();
+        const int item = s_item;
+        if (item >= n_items) break;
+        const int entry = item * 256 + threadIdx.x;
+        const bool valid = entry < n;
+        uint32_t s = 0, len = 0;
+        uint64_t base = 0;
+        uint32_t stride = 0;
+        if (valid) {
+            s = A.ovf_list[entry];
+            len = A.sorted_len[s];
+            const uint32_t blk = A.sorted_blk[s];
+            const BlockDesc bd = A.blocks[blk];
+            const uint32_t pos = s - bd.first;
+            if ((int)blk >= A.n_lane_blocks_first) {
+                base = bd.word_off + (uint64_t)(pos & 63) * 2 + (pos >> 6);
+                stride = 2 * kWave;
+            } else {
+                base = bd.word_off + pos;
+                stride = 2;
+            }
+        }
+        // rows of the wave = longest member, rounded to a multiple of 4
+        uint32_t lmax = len;
+        for (int off = 32; off > 0; off >>= 1) lmax = max(lmax, (uint32_t)__shfl_xor((int)lmax, off));
+        const int quads = (int)((lmax + 3) >> 2);
+        int S = kZero32;
+        for (int seg = 0; seg < nseg; ++seg) {
+            const int seg0 = seg * kSegCols;
+            const int segw = min(kSegCols, A.qpad - seg0);
+            if (seg != loaded) {
+                __syncthreads();
+                stage_profile(lds, A.prof, A.qpad, A.ps, seg0, segw);
+                __syncthreads();
+                loaded = seg;
+            }
+            for (int c0 = seg0; c0 < seg0 + segw; c0 += W) {
+                const bool has_in = c0 > 0;
+                const bool has_out = c0 + W < A.qpad;
+                int h[W], e[W], hq[W], M[W];
+#pragma unroll
+                for (int c = 0; c < W; ++c) {
+                    h[c] = kZero32;
+                    e[c] = kZero32;
+                    hq[c] = kZero32;
+                    M[c] = kZero32;
+                }
+                int diag0 = kZero32;
+                for (int k = 0; k < quads; ++k) {
+                    uint32_t w = 0x17171717u;   // pad code 23 beyond the sequence
+                    if ((uint32_t)(4 * k) < len) w = A.words[base + (uint64_t)k * stride];
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int row = 4 * k + r;
+                        int a = (w >> (8 * r)) & 0xff;
+                        if ((uint32_t)row >= len) a = 23;
+                        int P[W];
+                        single_profile<W>(P, lds, A.ps, a, c0 - seg0);
+                        uint2 bin = make_uint2((uint32_t)kZero32, (uint32_t)kZero32);
+                        if (has_in) bin = bord[(size_t)row * kWave + lane];
+                        int fout;
+                        row_cells<W, int>(h, e, hq, M, P, diag0, (int)bin.x, (int)bin.y,
+                                          A.gap_oe, A.gap_e, fout);
+                        if (has_out)
+                            bord[(size_t)row * kWave + lane] =
+                                make_uint2((uint32_t)h[W - 1], (uint32_t)fout);
+                        diag0 = (int)bin.x;
+                    }
+                }
+                S = vmax(S, reduce_max<W, int>(M));
+            }
+        }
+        if (valid) {
+            const uint32_t idx = A.perm[s];
+            if (idx != kNoSeq) A.scores[idx] = (int)((int64_t)S - (int64_t)kZero32);
+        }
+    }
+}
+
+__global__ void k_reset(uint32_t* p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+// ---------------------------------------------------------------------------
+hipError_t launch_reset(uint32_t* p, int n, hipStream_t s) {
+    hipLaunchKernelGGL(k_reset, dim3((n + 255) / 256), dim3(256), 0, s, p, n);
+    return hipGetLastError();
+}
+
+template <int W>
+static hipError_t launch_lane_w(const ScanArgs& a, int grid, size_t lds, hipStream_t s) {
+    hipLaunchKernelGGL(k_lane_scan<W>, dim3(grid), dim3(256), lds, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_lane_scan(int width, const ScanArgs& a, int grid, size_t lds, hipStream_t s) {
+    switch (width) {
+        case 16: return launch_lane_w<16>(a, grid, lds, s);
+        case 24: return launch_lane_w<24>(a, grid, lds, s);
+        case 32: return launch_lane_w<32>(a, grid, lds, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_rescore(const RescoreArgs& a, int grid, size_t lds, hipStream_t s) {
+    hipLaunchKernelGGL(k_rescore, dim3(grid), dim3(256), lds, s, a);
+    return hipGetLastError();
+}
+
+template <typename K>
+static int occ(K kern, size_t lds) {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kern, 256, lds) != hipSuccess) return 1;
+    return n > 0 ? n : 1;
+}
+
+int lane_scan_max_wg_per_cu(int width, size_t lds) {
+    switch (width) {
+        case 16: return occ(k_lane_scan<16>, lds);
+        case 24: return occ(k_lane_scan<24>, lds);
+        case 32: return occ(k_lane_scan<32>, lds);
+        default: return 1;
+    }
+}
+
+int rescore_max_wg_per_cu(size_t lds) { return occ(k_rescore, lds); }
+
+}  // namespace swh

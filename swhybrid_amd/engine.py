@@ -1,0 +1,135 @@
+"""Python front end of the HIP search library.
+
+    eng = Engine("blosum62", gap_open=10, gap_extend=2)
+    db = eng.load_db(seqdb)                 # packed and resident in HBM
+    scores, stats = eng.search(db, query)   # exact int32 scores, caller order
+    top = topk(scores, 20)                  # (score desc, index asc)
+
+All work happens in libswh.so; this module only marshals arrays.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _native as N
+from .alphabet import matrix as named_matrix
+from .datagen import SeqDB
+
+
+def _as_matrix(m) -> np.ndarray:
+    if isinstance(m, str):
+        m = named_matrix(m)
+    m = np.ascontiguousarray(np.asarray(m, dtype=np.int32))
+    if m.shape != (32, 32):
+        raise ValueError("matrix must be 32x32")
+    return m
+
+
+def _as_query(q) -> np.ndarray:
+    q = np.ascontiguousarray(np.asarray(q, dtype=np.uint8))
+    if q.ndim != 1:
+        raise ValueError("query must be a 1-D code array")
+    return q
+
+
+def _opts(strip_width=0, overflow_threshold=0, group_min_len=0, use_graph=0):
+    return N.SwhSearchOpts(int(strip_width), int(overflow_threshold), int(group_min_len),
+                           int(use_graph))
+
+
+class Database:
+    def __init__(self, engine: "Engine", handle, n: int, residues: int):
+        self._engine = engine
+        self.handle = handle
+        self.n = n
+        self.residues = residues
+
+    def info(self):
+        a, b, c = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        N.check(N.lib().swh_db_info(self.handle, C.byref(a), C.byref(b), C.byref(c)))
+        return {"nseq": a.value, "residues": b.value, "device_bytes": c.value}
+
+    def close(self):
+        if self.handle:
+            N.check(N.lib().swh_db_destroy(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Engine:
+    def __init__(self, matrix="blosum62", gap_open=10, gap_extend=2, device=0):
+        self.matrix = _as_matrix(matrix)
+        self.gap_open = int(gap_open)
+        self.gap_extend = int(gap_extend)
+        h = C.c_void_p()
+        N.check(N.lib().swh_ctx_create(int(device), N.ptr(self.matrix), self.gap_open,
+                                       self.gap_extend, C.byref(h)))
+        self.handle = h
+
+    def set_scoring(self, matrix, gap_open, gap_extend):
+        m = _as_matrix(matrix)
+        N.check(N.lib().swh_ctx_set_scoring(self.handle, N.ptr(m), int(gap_open),
+                                            int(gap_extend)))
+        self.matrix, self.gap_open, self.gap_extend = m, int(gap_open), int(gap_extend)
+
+    def load_db(self, db: SeqDB) -> Database:
+        res = np.ascontiguousarray(db.residues, dtype=np.uint8)
+        off = np.ascontiguousarray(db.offsets, dtype=np.uint64)
+        lens = np.ascontiguousarray(db.lengths, dtype=np.uint32)
+        h = C.c_void_p()
+        N.check(N.lib().swh_db_create(self.handle, N.ptr(res), N.ptr(off), N.ptr(lens),
+                                      db.n, C.byref(h)))
+        return Database(self, h, db.n, db.total)
+
+    def search(self, db: Database, query, **opts):
+        q = _as_query(query)
+        out = np.empty(db.n, dtype=np.int32)
+        st = N.SwhStats()
+        o = _opts(**opts)
+        N.check(N.lib().swh_search(self.handle, db.handle, N.ptr(q), len(q), C.byref(o),
+                                   N.ptr(out), C.byref(st)))
+        return out, st.as_dict()
+
+    def search_batch(self, db: Database, queries, **opts):
+        qs = [_as_query(q) for q in queries]
+        arr = (C.c_void_p * len(qs))(*[q.ctypes.data for q in qs])
+        lens = np.array([len(q) for q in qs], dtype=np.int32)
+        out = np.empty((len(qs), db.n), dtype=np.int32)
+        st = N.SwhStats()
+        o = _opts(**opts)
+        N.check(N.lib().swh_search_batch(self.handle, db.handle, arr, N.ptr(lens), len(qs),
+                                         C.byref(o), N.ptr(out), C.byref(st)))
+        return out, st.as_dict()
+
+    def close(self):
+        if self.handle:
+            N.check(N.lib().swh_ctx_destroy(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def topk(scores, k: int):
+    """Top-k of a score vector, (score desc, index asc), padded with (0, 0)."""
+    s = np.ascontiguousarray(scores, dtype=np.int32)
+    os_ = np.zeros(k, dtype=np.int32)
+    oi = np.zeros(k, dtype=np.int64)
+    N.check(N.lib().swh_topk(N.ptr(s), s.shape[0], int(k), N.ptr(os_), N.ptr(oi)))
+    return os_, oi
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    N.check(N.lib().swh_device_count(C.byref(n)))
+    return n.value

@@ -1,0 +1,132 @@
+"""GPU parity: libswh.so scores against the oracle, bit for bit.
+
+Every test goes through the C ABI (ctypes) onto cuda:0.  Sizes keep the CPU
+oracle within a few seconds.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from swhybrid_amd import alphabet as A
+from swhybrid_amd import datagen as G
+from swhybrid_amd.engine import Engine, topk
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def eng():
+    e = Engine("blosum62", 10, 2)
+    yield e
+    e.close()
+
+
+def small_db(n, seed, mean=300, lo=20, hi=2000):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    return G.SeqDB.from_lengths(rng, G.lognormal_lengths(rng, n, mean, 0.7, lo, hi))
+
+
+def check(eng, db, q, **opts):
+    d = eng.load_db(db)
+    got, st = eng.search(d, q, **opts)
+    want = O.search_i32(q, db, eng.matrix, eng.gap_open, eng.gap_extend)
+    bad = np.flatnonzero(got != want)
+    assert bad.size == 0, (f"{bad.size} mismatches, first {bad[:5]}: got {got[bad[:5]]} "
+                           f"want {want[bad[:5]]}")
+    d.close()
+    return got, st
+
+
+@pytest.mark.parametrize("width", [16, 24, 32])
+def test_c1_slice_all_widths(eng, width):
+    db = small_db(1500, 11)
+    q = G.c1_query()
+    _, st = check(eng, db, q, strip_width=width)
+    assert st["n_overflow"] == 0
+    assert st["cells"] == db.total * len(q)
+
+
+@pytest.mark.parametrize("qlen", [1, 2, 7, 33, 100, 1343, 1344, 1345, 2700])
+def test_query_lengths(eng, qlen):
+    db = small_db(300, 100 + qlen, mean=200, hi=800)
+    q = G.queries([qlen], 7 + qlen)[0]
+    check(eng, db, q)
+
+
+def test_ragged_and_tiny_sequences(eng):
+    rng = np.random.Generator(np.random.PCG64(5))
+    lens = [1, 2, 3, 4, 5, 7, 8, 9, 15, 16, 17, 63, 64, 65, 127, 128, 129, 300, 1, 1]
+    seqs = [G.random_residues(rng, n) for n in lens]
+    db = G.SeqDB.from_list(seqs * 9)          # 180 sequences: a partial last block
+    check(eng, db, G.queries([77], 3)[0])
+
+
+def test_single_sequence_and_empty(eng):
+    rng = np.random.Generator(np.random.PCG64(6))
+    db = G.SeqDB.from_list([G.random_residues(rng, 50)])
+    check(eng, db, G.queries([40], 4)[0])
+    empty = G.SeqDB.from_list([])
+    d = eng.load_db(empty)
+    got, st = eng.search(d, G.queries([40], 4)[0])
+    assert got.shape == (0,) and st["cells"] == 0
+
+
+def test_pad_and_unknown_codes(eng):
+    rng = np.random.Generator(np.random.PCG64(8))
+    seqs = [G.random_residues(rng, 120) for _ in range(130)]
+    for s in seqs[::3]:
+        s[::7] = 23                            # pad code inside a sequence
+    for s in seqs[1::3]:
+        s[::5] = 20                            # X
+    db = G.SeqDB.from_list(seqs)
+    q = G.queries([90], 9)[0]
+    q[::11] = 23
+    check(eng, db, q)
+    # codes above 23 in the database behave as 23 (src/Worker.cpp:150-154)
+    hi = G.SeqDB.from_list([np.where(s == 23, 200, s).astype(np.uint8) for s in seqs])
+    d = eng.load_db(hi)
+    got, _ = eng.search(d, q)
+    want = O.search_i32(q, db, eng.matrix, 10, 2)
+    assert (got == want).all()
+
+
+def test_forced_int32_rescore_matches(eng):
+    db = small_db(400, 21)
+    q = G.queries([200], 22)[0]
+    got, st = check(eng, db, q, overflow_threshold=1)
+    assert st["n_overflow"] == int((got >= 1).sum())
+
+
+def test_near_copies_blosum50(eng):
+    rng = np.random.Generator(np.random.PCG64(31))
+    q = G.queries([1500], 32)[0]
+    seqs = [G.mutate(rng, q) for _ in range(40)] + [G.random_residues(rng, 400) for _ in range(100)]
+    db = G.SeqDB.from_list(seqs)
+    e2 = Engine("blosum50", 13, 2)
+    got, st = check(e2, db, q)
+    assert got[:40].min() > 1000
+    got2, _ = check(e2, db, q, overflow_threshold=2000)
+    e2.close()
+
+
+def test_topk_matches_resultlist(eng):
+    db = small_db(2000, 41)
+    q = G.c1_query()
+    d = eng.load_db(db)
+    got, _ = eng.search(d, q)
+    s1, i1 = topk(got, 20)
+    s2, i2 = O.topk(O.search_i32(q, db, eng.matrix), 20)
+    assert (s1 == s2).all() and (i1 == i2).all()
+
+
+def test_reference_cascade_equals_gpu(eng):
+    """The reference-restating cascade (8-bit pass + int16 rescore) agrees too."""
+    rng = np.random.Generator(np.random.PCG64(51))
+    q = G.queries([464], 52)[0]
+    seqs = [G.mutate(rng, q) for _ in range(30)] + [G.random_residues(rng, 350) for _ in range(300)]
+    db = G.SeqDB.from_list(seqs)
+    d = eng.load_db(db)
+    got, _ = eng.search(d, q)
+    casc, ovf = O.search(q, db, eng.matrix, 10, 2)
+    assert ovf.sum() >= 30
+    assert (got == casc).all()

@@ -1,0 +1,206 @@
+"""The oracle, pinned: golden vectors from the reference's own kernels, brute
+force on tiny inputs, closed forms and invariants.  CPU only."""
+import glob
+import hashlib
+import itertools
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle as O
+from swhybrid_amd import alphabet as A
+from swhybrid_amd import datagen as G
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+REF_PARAM_H = "/root/reference/src/Param.h"
+M62 = A.BLOSUM62
+
+
+# --- golden vectors -----------------------------------------------------------
+
+def _regen(name):
+    from golden_inputs import regenerate  # tests/golden_inputs.py
+    return regenerate(name)
+
+
+@pytest.mark.parametrize("path", sorted(glob.glob(os.path.join(GOLDEN, "*.npz"))),
+                         ids=lambda p: os.path.basename(p))
+def test_oracle_matches_reference_golden(path):
+    f = np.load(path, allow_pickle=False)
+    name = os.path.basename(path)[:-4]
+    db, q = _regen(name)
+    h = hashlib.sha256()
+    h.update(np.ascontiguousarray(db.lengths).tobytes())
+    h.update(np.ascontiguousarray(db.residues).tobytes())
+    assert h.hexdigest() == str(f["residues_sha256"]), "generator drifted from the fixture"
+    assert np.array_equal(q, f["query"])
+    m = A.matrix(str(f["matrix"]))
+    go, ge = int(f["gap_open"]), int(f["gap_extend"])
+    exact = O.search_i32(q, db, m, go, ge)
+    casc, ovf = O.search(q, db, m, go, ge)
+    assert np.array_equal(exact, casc)
+    assert np.array_equal(ovf, f["overflow_avx2"])          # same int8 overflow set
+    assert np.array_equal(f["overflow_avx2"], f["overflow_sse"])
+    if bool(f["int8_only"]):
+        keep = f["overflow_avx2"] == 0
+        assert keep.sum() > 0
+        assert np.array_equal(exact[keep], f["scores_avx2"][keep])
+        assert np.array_equal(exact[keep], f["scores_sse"][keep])
+    else:
+        assert np.array_equal(exact, f["scores_avx2"])
+        assert np.array_equal(exact, f["scores_sse"])
+
+
+# --- brute force on tiny inputs -------------------------------------------------
+
+def brute_local(q, d, m, go, ge):
+    """Maximum over every local alignment, enumerated path by path."""
+    best = 0
+    nq, nd = len(q), len(d)
+
+    def walk(i, j, score, state):
+        # state: 0 after a match column, 1 inside a gap in q (d residue vs '-'),
+        # 2 inside a gap in d (q residue vs '-'); first gap residue costs go+ge
+        nonlocal best
+        best = max(best, score)
+        if i < nd and j < nq:
+            walk(i + 1, j + 1, score + int(m[q[j], d[i]]), 0)
+        if score > 0:  # a local alignment never starts or ends with a gap
+            if i < nd:
+                walk(i + 1, j, score - (ge if state == 1 else go + ge), 1)
+            if j < nq:
+                walk(i, j + 1, score - (ge if state == 2 else go + ge), 2)
+
+    for i in range(nd):
+        for j in range(nq):
+            walk(i + 1, j + 1, int(m[q[j], d[i]]), 0)
+    return best
+
+
+@pytest.mark.parametrize("seed", range(40))
+def test_gotoh_equals_brute_force(seed):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    nq, nd = int(rng.integers(1, 6)), int(rng.integers(1, 6))
+    go, ge = [(10, 2), (1, 1), (0, 1), (3, 0)][seed % 4]
+    # a small alphabet so that gapped alignments matter
+    alph = np.array([0, 2, 3, 19], dtype=np.uint8)
+    q = alph[rng.integers(0, 4, nq)]
+    d = alph[rng.integers(0, 4, nd)]
+    m = A.BLOSUM50 if seed % 2 else M62
+    assert O.gotoh(q, d, m, go, ge) == brute_local(q, d, m, go, ge)
+
+
+def test_brute_force_sees_gaps():
+    # WWWAWWW against WWWWWW: one gap of length 1 gives 6 W/W matches minus
+    # (open+extend); without the gap the best diagonal holds 5 W/W and one A/W.
+    q = np.array([19, 19, 19, 0, 19, 19, 19], dtype=np.uint8)
+    d = np.array([19] * 6, dtype=np.uint8)
+    ungapped = 5 * 11 - 3
+    for go, ge in [(1, 1), (10, 2), (20, 2)]:
+        want = max(6 * 11 - (go + ge), ungapped)
+        assert brute_local(q, d, M62, go, ge) == want
+        assert O.gotoh(q, d, M62, go, ge) == want
+
+
+# --- closed forms and invariants --------------------------------------------------
+
+def rand(rng, n):
+    return G.random_residues(rng, n)
+
+
+def test_single_residue_query_closed_form():
+    rng = np.random.Generator(np.random.PCG64(3))
+    for _ in range(50):
+        q = rand(rng, 1)
+        d = rand(rng, int(rng.integers(1, 60)))
+        want = max(0, max(int(M62[q[0], x]) for x in d))
+        assert O.gotoh(q, d, M62) == want
+
+
+def test_invariants():
+    rng = np.random.Generator(np.random.PCG64(4))
+    for t in range(60):
+        q = rand(rng, int(rng.integers(1, 80)))
+        d = rand(rng, int(rng.integers(1, 80)))
+        s = O.gotoh(q, d, M62)
+        assert 0 <= s <= 11 * min(len(q), len(d))
+        assert s == O.gotoh(d, q, M62)                                   # symmetric matrix
+        pad = np.full(int(rng.integers(1, 9)), 23, dtype=np.uint8)
+        assert s == O.gotoh(np.concatenate([q, pad]), d, M62)            # pad columns
+        assert s == O.gotoh(q, np.concatenate([pad, d, pad]), M62)       # pad rows
+        diag = sum(int(M62[c, c]) for c in q)
+        assert O.gotoh(q, q, M62) >= diag
+    # ungapped identity of a sequence with itself scores exactly the diagonal sum
+    q = np.array([19, 2, 7, 13], dtype=np.uint8)
+    assert O.gotoh(q, q, M62) == 11 + 9 + 8 + 7
+
+
+def test_gap_penalty_monotone():
+    rng = np.random.Generator(np.random.PCG64(9))
+    q = rand(rng, 120)
+    d = G.mutate(rng, q, sub=0.05, ins=0.03, dele=0.03)
+    prev = None
+    for go in (0, 2, 5, 10, 20):
+        s = O.gotoh(q, d, M62, go, 2)
+        if prev is not None:
+            assert s <= prev
+        prev = s
+
+
+def test_saturating_scans_agree_with_exact():
+    rng = np.random.Generator(np.random.PCG64(5))
+    lim = O.limit8(M62)
+    assert lim == 117 and O.limit8(A.BLOSUM50) == 113
+    for t in range(80):
+        q = rand(rng, int(rng.integers(1, 200)))
+        d = G.mutate(rng, q) if t % 3 == 0 else rand(rng, int(rng.integers(1, 200)))
+        exact = O.gotoh(q, d, M62)
+        s8 = O.sat_scan(q, d, M62, bits=8)
+        s16 = O.sat_scan(q, d, M62, bits=16)
+        assert s16 == exact                      # far below the int16 ceiling
+        if exact < lim:
+            assert s8 == exact                   # the 8-bit pass is exact below the limit
+        else:
+            assert s8 >= lim                     # and flags everything at or above it
+
+
+def test_topk_resultlist_semantics():
+    rng = np.random.Generator(np.random.PCG64(6))
+    s = rng.integers(-3, 40, size=5000).astype(np.int32)
+    sc, ix = O.topk(s, 25)
+    order = sorted(((-int(v), i) for i, v in enumerate(s) if v > 0))[:25]
+    assert list(sc) == [-a for a, _ in order]
+    assert list(ix) == [i for _, i in order]
+    sc, ix = O.topk(np.array([0, -1, 5, 0], dtype=np.int32), 3)
+    assert list(sc) == [5, 0, 0] and list(ix) == [2, 0, 0]              # (0, 0) padding
+
+
+# --- scoring tables ----------------------------------------------------------------
+
+def test_blosum_tables():
+    for m in (A.BLOSUM62, A.BLOSUM50):
+        assert (m == m.T).all()
+        assert (m[23:, :] == 0).all() and (m[:, 23:] == 0).all()
+    assert A.BLOSUM62[A.CODE_ORDER.index("W"), A.CODE_ORDER.index("W")] == 11
+    assert A.BLOSUM50[A.CODE_ORDER.index("W"), A.CODE_ORDER.index("W")] == 15
+    assert A.BLOSUM50[A.CODE_ORDER.index("C"), A.CODE_ORDER.index("C")] == 13
+
+
+@pytest.mark.skipif(not os.path.exists(REF_PARAM_H), reason="reference checkout not present")
+def test_blosum62_equals_reference_table():
+    text = open(REF_PARAM_H).read()
+    body = text[text.index("blosum62[32][32]"):text.index("struct cudaProp")]
+    vals = [int(v) for v in re.findall(r"-?\d+", body)][3:]   # skip "62", "32", "32"
+    ref = np.array(vals[:1024], dtype=np.int32).reshape(32, 32)
+    assert np.array_equal(ref, A.BLOSUM62)
+
+
+def test_encode_roundtrip():
+    s = "ARNDCQEGHILKMFPSTWYVBZX"
+    codes = A.encode(s)
+    assert A.decode(codes) == s
+    assert A.encode("@")[0] == 23
+    with pytest.raises(ValueError):
+        A.encode("A1")
