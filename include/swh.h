@@ -44,9 +44,12 @@ typedef struct swh_db swh_db;
 /* Timing and counters of the last search on a context. */
 typedef struct swh_stats {
     double kernel_ms;         /* int16 scan kernels (HIP events on the search stream) */
+    double group_ms;          /*   of which the group-mode scan (long sequences) */
+    double lane_ms;           /*   of which the lane-mode scan */
     double rescore_ms;        /* int32 rescore kernel */
     double total_ms;          /* whole call, host wall clock */
     uint64_t cells;           /* sum over sequences of len(db seq) * len(query) */
+    uint64_t cells_group;     /*   of which in group-mode (long) sequences */
     uint64_t padded_cells;    /* cells actually computed, padding included */
     uint64_t n_overflow;      /* sequences rescored in int32 */
     uint64_t n_launches;      /* kernel launches issued by the call */
@@ -58,11 +61,19 @@ typedef struct swh_search_opts {
                                  0 = auto, else 16, 24 or 32 */
     int overflow_threshold;   /* 0 = exact default (65535 - max(matrix));
                                  a lower value forces more int32 rescoring */
-    int group_min_len;        /* database length from which whole wavefronts
-                                 cooperate on one sequence pair; 0 = auto,
-                                 <0 = never */
+    int group_lanes;          /* lanes cooperating on one long sequence pair
+                                 (1..64); 0 = auto */
+    int group_width;          /* query columns per lane in group mode: 0 = auto,
+                                 else 8 or 16 */
     int use_graph;            /* batched calls: capture launches in a hipGraph */
 } swh_search_opts;
+
+/* Database layout options.  Zero-initialise for defaults. */
+typedef struct swh_db_opts {
+    uint32_t group_min_len;   /* sequences at least this long are stored as
+                                 pairs for group mode; 0 = default (2048),
+                                 0xffffffff = never */
+} swh_db_opts;
 
 const char* swh_last_error(void);
 int swh_version(void);
@@ -81,7 +92,8 @@ int swh_ctx_set_scoring(swh_ctx* ctx, const int32_t* matrix, int gap_open, int g
  * re-packed and re-copied every 256 MB chunk.  residues holds every sequence
  * back to back; sequence i is residues[offsets[i] .. offsets[i]+lengths[i]). */
 int swh_db_create(swh_ctx* ctx, const uint8_t* residues, const uint64_t* offsets,
-                  const uint32_t* lengths, uint64_t nseq, swh_db** out);
+                  const uint32_t* lengths, uint64_t nseq, const swh_db_opts* opts,
+                  swh_db** out);
 int swh_db_destroy(swh_db* db);
 /* nseq, total residues, device bytes held by the packed layout. */
 int swh_db_info(const swh_db* db, uint64_t* nseq, uint64_t* residues, uint64_t* device_bytes);

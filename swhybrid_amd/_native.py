@@ -27,9 +27,12 @@ EXPORTS = (
 class SwhStats(C.Structure):
     _fields_ = [
         ("kernel_ms", C.c_double),
+        ("group_ms", C.c_double),
+        ("lane_ms", C.c_double),
         ("rescore_ms", C.c_double),
         ("total_ms", C.c_double),
         ("cells", C.c_uint64),
+        ("cells_group", C.c_uint64),
         ("padded_cells", C.c_uint64),
         ("n_overflow", C.c_uint64),
         ("n_launches", C.c_uint64),
@@ -43,9 +46,14 @@ class SwhSearchOpts(C.Structure):
     _fields_ = [
         ("strip_width", C.c_int),
         ("overflow_threshold", C.c_int),
-        ("group_min_len", C.c_int),
+        ("group_lanes", C.c_int),
+        ("group_width", C.c_int),
         ("use_graph", C.c_int),
     ]
+
+
+class SwhDbOpts(C.Structure):
+    _fields_ = [("group_min_len", C.c_uint32)]
 
 
 class SwhError(RuntimeError):
@@ -72,7 +80,7 @@ def lib():
     L.swh_ctx_create.argtypes = [C.c_int, vp, C.c_int, C.c_int, C.POINTER(vp)]
     L.swh_ctx_destroy.argtypes = [vp]
     L.swh_ctx_set_scoring.argtypes = [vp, vp, C.c_int, C.c_int]
-    L.swh_db_create.argtypes = [vp, vp, vp, vp, C.c_uint64, C.POINTER(vp)]
+    L.swh_db_create.argtypes = [vp, vp, vp, vp, C.c_uint64, C.POINTER(SwhDbOpts), C.POINTER(vp)]
     L.swh_db_destroy.argtypes = [vp]
     L.swh_db_info.argtypes = [vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
                               C.POINTER(C.c_uint64)]
@@ -85,6 +93,30 @@ def lib():
         if name != "swh_last_error" and name != "swh_version":
             getattr(L, name).restype = C.c_int
     _lib = L
+    return L
+
+
+HOST_EXPORTS = ("swhy_last_error", "swhy_pool_search")
+_host = None
+
+
+def host_lib():
+    """Load libswhybrid_host.so, the C++ worker pipeline (DataPool, HIPWorker,
+    RecalcWorker, WorkerPool) built on top of libswh.so."""
+    global _host
+    if _host is not None:
+        return _host
+    lib()
+    if not os.path.exists(HOST_LIB_PATH):
+        raise SwhError(f"{HOST_LIB_PATH} is missing: build it with __graft_entry__.build()")
+    L = C.CDLL(HOST_LIB_PATH)
+    vp = C.c_void_p
+    L.swhy_last_error.restype = C.c_char_p
+    L.swhy_pool_search.restype = C.c_int
+    L.swhy_pool_search.argtypes = [vp, vp, vp, C.c_uint64, vp, C.c_int, vp, C.c_int, C.c_int,
+                                   C.c_int, C.c_uint64, C.c_int, vp, vp, vp, vp,
+                                   C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    _host = L
     return L
 
 

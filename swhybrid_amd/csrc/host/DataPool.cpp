@@ -1,0 +1,69 @@
+#include "DataPool.h"
+
+#include <algorithm>
+#include <cstring>
+#include <numeric>
+
+static const unsigned kBatchWidth = 128;   // src/Defs.h:15
+
+DataPool::DataPool(const uint8_t* residues, const uint64_t* offsets, const uint32_t* lengths,
+                   uint64_t nseq, const uint8_t* query, size_t qlen)
+    : residues_(residues), offsets_(offsets), lengths_(lengths), query_(query, query + qlen) {
+    order_.resize(nseq);
+    std::iota(order_.begin(), order_.end(), 0u);
+    std::stable_sort(order_.begin(), order_.end(),
+                     [&](uint32_t a, uint32_t b) { return lengths[a] > lengths[b]; });
+    uint64_t i = 0;
+    while (i < nseq) {
+        unsigned len = std::max<unsigned>(4, (lengths[order_[i]] + 3) & ~3u);
+        unsigned n = 0;
+        while (i + n < nseq && n < kBatchWidth &&
+               std::max<unsigned>(4, (lengths[order_[i + n]] + 3) & ~3u) == len)
+            ++n;
+        batches_.push_back(Batch{i, n, len});
+        totalPadded_ += (size_t)n * len;
+        maxBatchBytes_ = std::max(maxBatchBytes_, (size_t)n * len);
+        i += n;
+    }
+}
+
+size_t DataPool::maxInfoPerChunk(size_t bufferSize) const {
+    // every batch holds at least 4 bytes per sequence and one sequence
+    return std::min(batches_.size(), bufferSize / 4 + 1);
+}
+
+void DataPool::rewind() {
+    std::lock_guard<std::mutex> g(mu_);
+    next_ = 0;
+}
+
+bool DataPool::fillChunk(char* dst, batchInfo* info, size_t& infoSize, size_t& aaSize,
+                         size_t bufferSize, bool /*cpuFlag*/, size_t& baseIndex) {
+    std::lock_guard<std::mutex> g(mu_);
+    infoSize = 0;
+    aaSize = 0;
+    if (next_ >= batches_.size()) return false;
+    baseIndex = batches_[next_].first;
+    size_t used = 0;
+    while (next_ < batches_.size()) {
+        const Batch& b = batches_[next_];
+        const size_t bytes = (size_t)b.num * b.len;
+        if (used + bytes > bufferSize && infoSize > 0) break;
+        if (bytes > bufferSize) throw "error: readBufferSize too small\n";
+        char* out = dst + used;
+        for (unsigned k = 0; k < b.num; ++k) {
+            const uint32_t id = order_[b.first + k];
+            const uint32_t n = lengths_[id];
+            const uint8_t* src = residues_ + offsets_[id];
+            for (uint32_t r = 0; r < b.len; ++r) {
+                uint8_t c = r < n ? src[r] : (uint8_t)kPadCode;
+                out[(size_t)k * b.len + r] = (char)(c > kPadCode ? kPadCode : c);
+            }
+        }
+        info[infoSize++] = batchInfo(b.num, b.len);
+        used += bytes;
+        ++next_;
+    }
+    aaSize = used;
+    return true;
+}

@@ -1,0 +1,91 @@
+#include "HIPWorker.h"
+
+#include <cstdio>
+#include <string>
+
+namespace {
+thread_local std::string g_msg;
+const char* swh_fail(const char* what) {
+    g_msg = std::string("HIPWorker: ") + what + ": " + swh_last_error() + "\n";
+    return g_msg.c_str();
+}
+}  // namespace
+
+HIPWorker::HIPWorker(DataPool* loader, Param* params, RecalcWorker* recalcWorker, int deviceId,
+                     size_t chunkBytes)
+    : Worker(loader, params, recalcWorker, 4, 128, sizeof(int)), deviceId_(deviceId) {
+    bufferSize = chunkBytes;
+}
+
+HIPWorker::~HIPWorker() { free(); }
+
+void HIPWorker::setQuery(const char* q, size_t qlen) {
+    Worker::setQuery(q, qlen);
+    for (size_t j = 0; j < qlen; ++j)
+        if ((unsigned char)q[j] > 23) throw "HIPWorker: query code above 23\n";
+}
+
+void HIPWorker::alloc() {
+    if (swh_ctx_create(deviceId_, params->plain_matrix, params->gapOpen, params->gapExtend,
+                       &ctx_) != SWH_OK)
+        throw swh_fail("swh_ctx_create");
+    for (int i = 0; i < 2; ++i) entry[i].setEntry(nullptr, nullptr, nullptr);
+}
+
+void HIPWorker::free() {
+    for (auto& d : dbs_) {
+        if (d) swh_db_destroy(d);
+        d = nullptr;
+    }
+    if (ctx_) swh_ctx_destroy(ctx_);
+    ctx_ = nullptr;
+}
+
+// Chunk -> resident packed database in the aux entry's slot.
+void HIPWorker::packDB() {
+    const int slot = (int)!bufFlag;
+    Entry* e = entry + slot;
+    TotalAminoAcidResidue += filledSize;
+    offsets_.clear();
+    lengths_.clear();
+    uint64_t off = 0;
+    for (size_t b = 0; b < infoSize; ++b)
+        for (unsigned k = 0; k < info[b].numSeqs; ++k) {
+            offsets_.push_back(off);
+            lengths_.push_back(info[b].seqLen);
+            off += info[b].seqLen;
+        }
+    if (dbs_[slot]) swh_db_destroy(dbs_[slot]);
+    dbs_[slot] = nullptr;
+    if (swh_db_create(ctx_, reinterpret_cast<const uint8_t*>(dbSeqBuf), offsets_.data(),
+                      lengths_.data(), lengths_.size(), nullptr, &dbs_[slot]) != SWH_OK)
+        throw swh_fail("swh_db_create");
+    scores_[slot].resize(lengths_.size());
+    e->setEntry(dbSeqBuf, nullptr, scores_[slot].data());
+    e->setBaseIdx(fillBaseIndex);
+    e->setBatchNum((int)infoSize);
+    e->setResultSize(lengths_.size());
+}
+
+void HIPWorker::search() {
+    const int slot = (int)bufFlag;
+    swh_stats st{};
+    if (swh_search(ctx_, dbs_[slot], reinterpret_cast<const uint8_t*>(query.data()), queryLen,
+                   nullptr, compEntry->getScores(), &st) != SWH_OK)
+        throw swh_fail("swh_search");
+    kernelMs_ += st.kernel_ms + st.rescore_ms;
+    cells_ += st.cells;
+    if (sink_) {
+        const int* s = compEntry->getScores();
+        for (size_t i = 0; i < compEntry->getResultSize(); ++i)
+            sink_[compEntry->getBaseIdx() + i] = s[i];
+    }
+}
+
+void HIPWorker::showPerformance() {
+    const double t = getStopTime() - getStartTime();
+    std::printf("HIP[%d] computing time %lfs, AARes %zu, GCUPs %lf\n", deviceId_, t,
+                TotalAminoAcidResidue, t > 0 ? TotalAminoAcidResidue * (double)queryLen / t / 1e9 : 0.0);
+}
+
+void HIPWorker::who() { std::printf("HIP\n"); }

@@ -1,0 +1,57 @@
+#include "WorkerPool.h"
+
+#include <algorithm>
+
+WorkerPool::WorkerPool(DataPool* loader, Param* params, size_t chunkBytes)
+    : loader_(loader), params_(params), resultList_(params->topNum) {
+    recalcWorker_ = new RecalcWorker(params, 0);
+    recalcWorker_->setQuery(loader->getQuery(), loader->getQueryLen());
+    for (int i = 0; i < params->hipNum; ++i)
+        workerList_.push_back(new HIPWorker(loader, params, recalcWorker_, i, chunkBytes));
+}
+
+WorkerPool::~WorkerPool() {
+    for (Worker* w : workerList_) delete w;
+    delete recalcWorker_;
+}
+
+int WorkerPool::run(int* scoreSink) {
+    if (workerList_.empty()) {
+        error_ = "no computing device";
+        return -1;
+    }
+    for (Worker* w : workerList_)
+        if (scoreSink) static_cast<HIPWorker*>(w)->setScoreSink(scoreSink);
+    for (Worker* w : workerList_) w->startThread();
+    for (Worker* w : workerList_) w->waitForThread();
+    for (Worker* w : workerList_)
+        if (!w->lastError().empty()) {
+            error_ = w->lastError();
+            return -1;
+        }
+    try {
+        recalcWorker_->launch();
+    } catch (const char* msg) {
+        error_ = msg;
+        return -1;
+    }
+    double start = 1e300, stop = 0;
+    for (Worker* w : workerList_) {
+        if (w->getStartTime() > 0) start = std::min(start, w->getStartTime());
+        stop = std::max(stop, w->getStopTime());
+    }
+    if (recalcWorker_->getNum() > 0) stop = std::max(stop, recalcWorker_->getStopTime());
+    seconds_ = stop > start ? stop - start : 0;
+    gcups_ = seconds_ > 0
+                 ? (double)loader_->getTotalResidue() * loader_->getQueryLen() / seconds_ / 1e9
+                 : 0;
+    for (Worker* w : workerList_) resultList_.mergeList(w->getResult());
+    resultList_.mergeList(recalcWorker_->getResult());
+    return 0;
+}
+
+double WorkerPool::kernelSeconds() const {
+    double s = 0;
+    for (Worker* w : workerList_) s += static_cast<HIPWorker*>(w)->kernelSeconds();
+    return s;
+}

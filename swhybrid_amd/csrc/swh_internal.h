@@ -40,8 +40,33 @@ struct ScanArgs {
     int blk_begin, blk_end;    // block range of this launch
     int n_items;               // work items (4 blocks each)
     int qlen, qpad, ps;        // query length, padded length, LDS row stride (int16)
+    int pstride;               // row stride of prof (int16), >= qpad
     int gap_oe, gap_e;
     int ovf_threshold;         // score >= threshold -> int32 rescore
+};
+
+// Group mode: G lanes cooperate on one sequence pair.  Lane g of a group holds
+// query columns [pass*G*W + g*W, +W) and works on row t-g at step t; the H/F
+// of its last column reach lane g+1 one step later through a wave-wide DPP
+// shift.  A wave holds floor(64/G) groups; a workgroup item is 4 waves.
+struct GroupArgs {
+    const uint32_t* words;
+    const BlockDesc* blocks;   // pair blocks (2 sequences each)
+    const uint32_t* perm;
+    const int16_t* prof;
+    int32_t* scores;
+    uint32_t* ovf_list;
+    uint32_t* ovf_count;
+    uint32_t* work_counter;
+    uint2* border;             // per wave slot: groups_per_wave * border_rows uint2
+    uint64_t border_stride;    // uint2 per wave slot
+    uint32_t border_rows;      // rows reserved per group (>= longest pair block)
+    int blk_begin, blk_end;    // pair-block range
+    int group;                 // G, lanes per group (1..64)
+    int n_items;
+    int qlen, qpad, ps, pstride;
+    int gap_oe, gap_e;
+    int ovf_threshold;
 };
 
 struct RescoreArgs {
@@ -58,7 +83,7 @@ struct RescoreArgs {
     uint2* border;
     uint64_t border_stride;
     int n_lane_blocks_first;      // index of the first lane-mode block
-    int qpad, ps;
+    int qpad, ps, pstride;
     int gap_oe, gap_e;
 };
 
@@ -66,6 +91,9 @@ struct RescoreArgs {
 hipError_t launch_reset(uint32_t* p, int n, hipStream_t s);
 hipError_t launch_lane_scan(int width, const ScanArgs& a, int grid, size_t lds_bytes,
                             hipStream_t s);
+hipError_t launch_group_scan(int width, const GroupArgs& a, int grid, size_t lds_bytes,
+                             hipStream_t s);
+int group_scan_max_wg_per_cu(int width, size_t lds_bytes);
 hipError_t launch_rescore(const RescoreArgs& a, int grid, size_t lds_bytes, hipStream_t s);
 int lane_scan_max_wg_per_cu(int width, size_t lds_bytes);
 int rescore_max_wg_per_cu(size_t lds_bytes);

@@ -44,13 +44,13 @@ constexpr int kZero32 = INT32_MIN;             // biased zero, int32 rescore
 
 // Copy profile rows 0..23, columns [seg0, seg0+segw) into LDS rows of stride ps.
 __device__ __forceinline__ void stage_profile(short* lds, const int16_t* __restrict__ prof,
-                                              int qpad, int ps, int seg0, int segw) {
+                                              int pstride, int ps, int seg0, int segw) {
     const int per_row = segw >> 3;
     const int total = kProfRows * per_row;
     for (int t = threadIdx.x; t < total; t += blockDim.x) {
         const int r = t / per_row;
         const int c8 = t - r * per_row;
-        const uint4 v = *reinterpret_cast<const uint4*>(prof + (size_t)r * qpad + seg0 + c8 * 8);
+        const uint4 v = *reinterpret_cast<const uint4*>(prof + (size_t)r * pstride + seg0 + c8 * 8);
         *reinterpret_cast<uint4*>(lds + r * ps + c8 * 8) = v;
     }
 }
@@ -203,7 +203,7 @@ __global__ __launch_bounds__(256) void k_lane_scan(ScanArgs A) {
             const int segw = min(kSegCols, A.qpad - seg0);
             if (seg != loaded) {
                 __syncthreads();
-                stage_profile(lds, A.prof, A.qpad, A.ps, seg0, segw);
+                stage_profile(lds, A.prof, A.pstride, A.ps, seg0, segw);
                 __syncthreads();
                 loaded = seg;
             }
@@ -216,6 +216,131 @@ __global__ __launch_bounds__(256) void k_lane_scan(ScanArgs A) {
         if (active) {
             emit(A, bd.first + lane, (int)S.x + 32768);
             emit(A, bd.first + kWave + lane, (int)S.y + 32768);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Group mode for long database sequences.  G lanes share one sequence pair:
+// lane g of a group keeps query columns [pass*G*W + g*W, +W) in registers and
+// at step t works on row t-g; the H and F of its last column reach lane g+1
+// one step later through a wave-wide DPP shift (wave_shr:1), so the pair is
+// swept as a pipeline along the database with one row of skew per lane.  The
+// first lane of a group reads the previous pass's right edge from the border
+// array, the last active lane writes it for the next pass.
+__device__ __forceinline__ uint32_t wave_shr1(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x138, 0xf, 0xf, false);
+}
+
+template <int W>
+__global__ __launch_bounds__(256) void k_group_scan(GroupArgs A) {
+    extern __shared__ __align__(16) short lds[];
+    __shared__ int s_item;
+    __shared__ uint32_t s_red[kWavesPerWG][kWave];
+    const int wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int G = A.group;
+    const int gpw = kWave / G;
+    const int gi = lane / G;
+    const int gl = lane - gi * G;
+    const bool in_group = gi < gpw;
+    uint2* bord = A.border + (size_t)(blockIdx.x * kWavesPerWG + wave) * A.border_stride +
+                  (size_t)(in_group ? gi : 0) * A.border_rows;
+    const s2 Q = {(short)A.gap_oe, (short)A.gap_oe};
+    const s2 R = {(short)A.gap_e, (short)A.gap_e};
+    const s2 Z = as_s2(kZero16x2);
+    const int pass_cols = G * W;
+    const int npass = (A.qpad + pass_cols - 1) / pass_cols;
+    for (;;) {
+        __syncthreads();
+        if (threadIdx.x == 0) s_item = (int)atomicAdd(A.work_counter, 1u);
+        __syncthreads();
+        const int item = s_item;
+        if (item >= A.n_items) break;
+        const int p = A.blk_begin + (item * kWavesPerWG + wave) * gpw + gi;
+        const bool has_pair = in_group && p < A.blk_end;
+        BlockDesc bd = {0, 0, 0};
+        if (has_pair) bd = A.blocks[p];
+        uint32_t rmax = bd.rows;
+        for (int off = 32; off > 0; off >>= 1)
+            rmax = max(rmax, (uint32_t)__shfl_xor((int)rmax, off));
+        rmax = __builtin_amdgcn_readfirstlane(rmax);   // equal in every lane: keep loops scalar
+        const uint2* pw = reinterpret_cast<const uint2*>(A.words + bd.word_off);
+        s2 S = Z;
+        for (int pass = 0; pass < npass; ++pass) {
+            const int col_base = pass * pass_cols;
+            const int pcols = min(pass_cols, A.qpad - col_base);
+            const int nl = pcols / W;
+            __syncthreads();
+            stage_profile(lds, A.prof, A.pstride, A.ps, col_base, pcols);
+            __syncthreads();
+            const bool has_in = pass > 0;
+            const bool has_out = pass + 1 < npass;
+            const bool on = has_pair && gl < nl;
+            const int jj0 = gl * W;
+            s2 h[W], e[W], hq[W], M[W];
+#pragma unroll
+            for (int c = 0; c < W; ++c) {
+                h[c] = Z;
+                e[c] = Z;
+                hq[c] = Z;
+                M[c] = Z;
+            }
+            s2 diag0 = Z;
+            uint32_t ch = kZero16x2, cf = kZero16x2;   // this lane's last (H, F)
+            const int steps = rmax > 0 ? (int)rmax + nl - 1 : 0;
+            for (int t = 0; t < steps; ++t) {
+                uint2 bin = make_uint2(kZero16x2, kZero16x2);
+                if (has_in && gl == 0 && has_pair && t < (int)bd.rows) bin = bord[t];
+                // every lane takes part in the shift (no lane may be masked off here)
+                const uint32_t hx = wave_shr1(ch);
+                const uint32_t fx = wave_shr1(cf);
+                const uint32_t hl_u = gl == 0 ? bin.x : hx;
+                const uint32_t fl_u = gl == 0 ? bin.y : fx;
+                const int row = t - gl;
+                if (on && row >= 0 && row < (int)bd.rows) {
+                    const uint2 w = pw[row >> 2];
+                    const int sh = 8 * (row & 3);
+                    const int a = (w.x >> sh) & 0xff;
+                    const int b = (w.y >> sh) & 0xff;
+                    s2 P[W];
+                    pair_profile<W>(P, lds, A.ps, a, b, jj0);
+                    s2 fout;
+                    row_cells<W, s2>(h, e, hq, M, P, diag0, as_s2(hl_u), as_s2(fl_u), Q, R, fout);
+                    ch = bits(h[W - 1]);
+                    cf = bits(fout);
+                    diag0 = as_s2(hl_u);
+                    if (has_out && gl == nl - 1) bord[row] = make_uint2(ch, cf);
+                }
+            }
+            if (on) S = vmax(S, reduce_max<W, s2>(M));
+        }
+        s_red[wave][lane] = bits(S);
+        __syncthreads();
+        if (has_pair && gl == 0) {
+            s2 m = S;
+            for (int k = 1; k < G; ++k) m = vmax(m, as_s2(s_red[wave][lane + k]));
+            const uint32_t s0 = bd.first;
+            const uint32_t idx0 = A.perm[s0];
+            const uint32_t idx1 = A.perm[s0 + 1];
+            const int sc0 = (int)m.x + 32768;
+            const int sc1 = (int)m.y + 32768;
+            if (idx0 != kNoSeq) {
+                if (sc0 >= A.ovf_threshold) {
+                    A.ovf_list[atomicAdd(A.ovf_count, 1u)] = s0;
+                    A.scores[idx0] = -1;
+                } else {
+                    A.scores[idx0] = sc0;
+                }
+            }
+            if (idx1 != kNoSeq) {
+                if (sc1 >= A.ovf_threshold) {
+                    A.ovf_list[atomicAdd(A.ovf_count, 1u)] = s0 + 1;
+                    A.scores[idx1] = -1;
+                } else {
+                    A.scores[idx1] = sc1;
+                }
+            }
         }
     }
 }
@@ -270,6 +395,7 @@ __global__ __launch_bounds__(256) void k_rescore(RescoreArgs A) {
         // rows of the wave = longest member, rounded to a multiple of 4
         uint32_t lmax = len;
         for (int off = 32; off > 0; off >>= 1) lmax = max(lmax, (uint32_t)__shfl_xor((int)lmax, off));
+        lmax = __builtin_amdgcn_readfirstlane(lmax);
         const int quads = (int)((lmax + 3) >> 2);
         int S = kZero32;
         for (int seg = 0; seg < nseg; ++seg) {
@@ -277,7 +403,7 @@ __global__ __launch_bounds__(256) void k_rescore(RescoreArgs A) {
             const int segw = min(kSegCols, A.qpad - seg0);
             if (seg != loaded) {
                 __syncthreads();
-                stage_profile(lds, A.prof, A.qpad, A.ps, seg0, segw);
+                stage_profile(lds, A.prof, A.pstride, A.ps, seg0, segw);
                 __syncthreads();
                 loaded = seg;
             }
@@ -350,6 +476,20 @@ hipError_t launch_lane_scan(int width, const ScanArgs& a, int grid, size_t lds, 
     }
 }
 
+template <int W>
+static hipError_t launch_group_w(const GroupArgs& a, int grid, size_t lds, hipStream_t s) {
+    hipLaunchKernelGGL(k_group_scan<W>, dim3(grid), dim3(256), lds, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_group_scan(int width, const GroupArgs& a, int grid, size_t lds, hipStream_t s) {
+    switch (width) {
+        case 8: return launch_group_w<8>(a, grid, lds, s);
+        case 16: return launch_group_w<16>(a, grid, lds, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
 hipError_t launch_rescore(const RescoreArgs& a, int grid, size_t lds, hipStream_t s) {
     hipLaunchKernelGGL(k_rescore, dim3(grid), dim3(256), lds, s, a);
     return hipGetLastError();
@@ -367,6 +507,14 @@ int lane_scan_max_wg_per_cu(int width, size_t lds) {
         case 16: return occ(k_lane_scan<16>, lds);
         case 24: return occ(k_lane_scan<24>, lds);
         case 32: return occ(k_lane_scan<32>, lds);
+        default: return 1;
+    }
+}
+
+int group_scan_max_wg_per_cu(int width, size_t lds) {
+    switch (width) {
+        case 8: return occ(k_group_scan<8>, lds);
+        case 16: return occ(k_group_scan<16>, lds);
         default: return 1;
     }
 }

@@ -34,9 +34,9 @@ def _as_query(q) -> np.ndarray:
     return q
 
 
-def _opts(strip_width=0, overflow_threshold=0, group_min_len=0, use_graph=0):
-    return N.SwhSearchOpts(int(strip_width), int(overflow_threshold), int(group_min_len),
-                           int(use_graph))
+def _opts(strip_width=0, overflow_threshold=0, group_lanes=0, group_width=0, use_graph=0):
+    return N.SwhSearchOpts(int(strip_width), int(overflow_threshold), int(group_lanes),
+                           int(group_width), int(use_graph))
 
 
 class Database:
@@ -79,13 +79,16 @@ class Engine:
                                             int(gap_extend)))
         self.matrix, self.gap_open, self.gap_extend = m, int(gap_open), int(gap_extend)
 
-    def load_db(self, db: SeqDB) -> Database:
+    def load_db(self, db: SeqDB, group_min_len: int = 0) -> Database:
+        """Pack and upload; sequences of at least group_min_len residues are
+        stored as pairs for group mode (0 = default 2048, -1 = never)."""
         res = np.ascontiguousarray(db.residues, dtype=np.uint8)
         off = np.ascontiguousarray(db.offsets, dtype=np.uint64)
         lens = np.ascontiguousarray(db.lengths, dtype=np.uint32)
         h = C.c_void_p()
+        o = N.SwhDbOpts(0xFFFFFFFF if group_min_len < 0 else int(group_min_len))
         N.check(N.lib().swh_db_create(self.handle, N.ptr(res), N.ptr(off), N.ptr(lens),
-                                      db.n, C.byref(h)))
+                                      db.n, C.byref(o), C.byref(h)))
         return Database(self, h, db.n, db.total)
 
     def search(self, db: Database, query, **opts):
@@ -133,3 +136,30 @@ def device_count() -> int:
     n = C.c_int(0)
     N.check(N.lib().swh_device_count(C.byref(n)))
     return n.value
+
+
+def pool_search(db: SeqDB, query, matrix="blosum62", gap_open=10, gap_extend=2, k=10,
+                chunk_bytes=0, n_workers=1, all_scores=False):
+    """Run one query through the C++ worker pipeline (DataPool -> HIPWorker
+    threads -> RecalcWorker -> merged top-k), as the reference's main() does.
+    Returns dict(top_score, top_index, top_pool_index, scores, seconds, gcups)."""
+    m = _as_matrix(matrix)
+    q = _as_query(query)
+    res = np.ascontiguousarray(db.residues, dtype=np.uint8)
+    off = np.ascontiguousarray(db.offsets, dtype=np.uint64)
+    lens = np.ascontiguousarray(db.lengths, dtype=np.uint32)
+    ts = np.zeros(k, dtype=np.int32)
+    ti = np.zeros(k, dtype=np.int64)
+    tp = np.zeros(k, dtype=np.int64)
+    sc = np.zeros(db.n, dtype=np.int32) if all_scores else None
+    secs, gc = C.c_double(0), C.c_double(0)
+    L = N.host_lib()
+    rc = L.swhy_pool_search(N.ptr(res), N.ptr(off), N.ptr(lens), db.n, N.ptr(q), len(q), N.ptr(m),
+                            int(gap_open), int(gap_extend), int(k), int(chunk_bytes),
+                            int(n_workers), N.ptr(ts), N.ptr(ti), N.ptr(tp),
+                            None if sc is None else N.ptr(sc), C.byref(secs), C.byref(gc))
+    if rc != 0:
+        msg = L.swhy_last_error()
+        raise N.SwhError(f"pool search failed ({rc}): {msg.decode() if msg else ''}")
+    return {"top_score": ts, "top_index": ti, "top_pool_index": tp, "scores": sc,
+            "seconds": secs.value, "gcups": gc.value}

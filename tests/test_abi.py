@@ -1,0 +1,55 @@
+"""The C ABI library loads and exports every symbol include/swh.h declares;
+host-only entry points behave without a GPU.  CPU only."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+
+import oracle as O
+from swhybrid_amd import _native as N
+from swhybrid_amd.engine import topk
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions(name):
+    text = open(os.path.join(ROOT, "include", name)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return set(re.findall(r"\b(swh\w+)\s*\(", text))
+
+
+def test_header_and_exports_agree():
+    declared = header_functions("swh.h")
+    assert declared == set(N.EXPORTS)
+    lib = N.lib()
+    for name in declared:
+        assert hasattr(lib, name), name
+
+
+def test_no_device_is_a_clean_error():
+    n = C.c_int(-1)
+    assert N.lib().swh_device_count(C.byref(n)) == 0
+    if n.value == 0:
+        h = C.c_void_p()
+        m = np.zeros(1024, dtype=np.int32)
+        rc = N.lib().swh_ctx_create(0, N.ptr(m), 10, 2, C.byref(h))
+        assert rc != 0 and not h.value
+        assert b"device" in N.lib().swh_last_error()
+
+
+def test_bad_arguments_are_rejected():
+    h = C.c_void_p()
+    bad = np.full(1024, 200, dtype=np.int32)
+    assert N.lib().swh_ctx_create(0, N.ptr(bad), 10, 2, C.byref(h)) != 0
+    assert N.lib().swh_ctx_create(0, None, 10, 2, C.byref(h)) != 0
+    assert N.lib().swh_topk(None, 5, 3, None, None) != 0
+
+
+def test_host_topk_matches_resultlist():
+    rng = np.random.Generator(np.random.PCG64(1))
+    for n, k, hi in [(10000, 20, 60), (50, 10, 3), (7, 10, 5), (0, 4, 1)]:
+        s = rng.integers(-5, hi, size=n).astype(np.int32)
+        a = topk(s, k)
+        b = O.topk(s, k)
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])

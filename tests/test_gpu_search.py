@@ -130,3 +130,49 @@ def test_reference_cascade_equals_gpu(eng):
     casc, ovf = O.search(q, db, eng.matrix, 10, 2)
     assert ovf.sum() >= 30
     assert (got == casc).all()
+
+
+# --- group mode (long sequences, lanes cooperating on a pair) ------------------
+
+def long_db(seed, n=60, lo=300, hi=5000):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    lens = G.loguniform_lengths(rng, n, lo, hi)
+    return G.SeqDB.from_lengths(rng, lens)
+
+
+def check_db(eng, db, q, min_len, **opts):
+    d = eng.load_db(db, group_min_len=min_len)
+    got, st = eng.search(d, q, **opts)
+    want = O.search_i32(q, db, eng.matrix, eng.gap_open, eng.gap_extend)
+    bad = np.flatnonzero(got != want)
+    assert bad.size == 0, (f"{bad.size} mismatches (opts {opts}), first {bad[:5]}: "
+                           f"got {got[bad[:5]]} want {want[bad[:5]]} len {db.lengths[bad[:5]]}")
+    d.close()
+    return got, st
+
+
+@pytest.mark.parametrize("qlen", [1, 50, 144, 700, 1100, 2700])
+def test_group_mode_auto(eng, qlen):
+    db = long_db(200 + qlen)
+    q = G.queries([qlen], 300 + qlen)[0]
+    check_db(eng, db, q, min_len=600)
+
+
+@pytest.mark.parametrize("lanes,width", [(1, 8), (3, 16), (7, 8), (16, 16), (63, 8), (64, 16),
+                                         (64, 8)])
+def test_group_mode_shapes(eng, lanes, width):
+    db = long_db(400 + lanes, n=40)
+    q = G.queries([1234], 401)[0]
+    check_db(eng, db, q, min_len=300, group_lanes=lanes, group_width=width)
+
+
+def test_group_mode_odd_count_and_overflow(eng):
+    rng = np.random.Generator(np.random.PCG64(61))
+    q = G.queries([900], 62)[0]
+    seqs = [G.mutate(rng, q) for _ in range(7)] + [G.random_residues(rng, int(n))
+                                                  for n in rng.integers(10, 3000, size=91)]
+    db = G.SeqDB.from_list(seqs)
+    check_db(eng, db, q, min_len=1000)                        # odd number of long sequences
+    got, st = check_db(eng, db, q, min_len=1000, overflow_threshold=500)
+    assert st["n_overflow"] >= 7
+    check_db(eng, db, q, min_len=1)                           # everything in pairs
