@@ -123,8 +123,8 @@ struct swh_ctx {
     hipStream_t stream = nullptr;
     int32_t matrix[1024];
     int gap_open = 10, gap_extend = 2, max_score = 0;
-    DevBuf border, counters, prof, scores, ovf;
-    PinnedBuf h_prof, h_scores;
+    DevBuf border, counters, prof, scores, ovf, desc;
+    PinnedBuf h_prof, h_scores, h_desc, h_ctr;
     hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     size_t border_budget = size_t(8) << 30;   // bytes of HBM for strip borders
 };
@@ -204,6 +204,9 @@ int swh_ctx_destroy(swh_ctx* c) {
     c->ovf.release();
     c->h_prof.release();
     c->h_scores.release();
+    c->desc.release();
+    c->h_desc.release();
+    c->h_ctr.release();
     for (auto& e : c->ev)
         if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -361,216 +364,275 @@ int swh_db_info(const swh_db* db, uint64_t* nseq, uint64_t* residues, uint64_t* 
     return SWH_OK;
 }
 
-static int pick_width(int qlen, int requested) {
-    if (requested == 16 || requested == 24 || requested == 32) return requested;
-    // smallest padding; ties go to the narrower strip (more waves per SIMD)
-    int best = 16, best_pad = ((qlen + 15) / 16) * 16;
-    for (int w : {24, 32}) {
-        int pad = ((qlen + w - 1) / w) * w;
-        if (pad < best_pad) {
-            best = w;
-            best_pad = pad;
-        }
-    }
-    return best;
+// Group mode shape for one query: lanes per group G for a given column width
+// W.  A group covers the whole query in one pass when it can (G = strips, at
+// most 64); the score is the useful fraction of lane-columns times the share
+// of a lane's step spent on cells (about 13 VALU ops of per-step overhead
+// beside the 9.5 per packed cell).
+static double group_eff(int qlen, int W, int req_g, int* G_out) {
+    const int qp = ((qlen + W - 1) / W) * W;
+    const int strips = qp / W;
+    const int G = req_g > 0 ? std::min(req_g, kWave) : std::min(strips, kWave);
+    const int passes = (strips + G - 1) / G;
+    const int gpw = kWave / G;
+    const double lane_use =
+        (double)strips / ((double)passes * G) * (double)(gpw * G) / kWave;
+    *G_out = G;
+    return lane_use * (9.0 * W) / (9.5 * W + 13.0) * ((double)qlen / qp);
 }
 
-// Group mode shape: columns per lane W and lanes per group G.  Prefers the
-// choice that leaves the fewest idle lane-columns, counting the per-step
-// overhead of a lane (about 13 VALU ops beside the 9.5 per packed cell).
-static void pick_group(int qlen, int req_w, int req_g, int* W_out, int* G_out) {
-    double best = -1;
-    int bw = 16, bg = 64;
-    for (int W : {16, 8}) {
-        if (req_w && req_w != W) continue;
-        const int qp = ((qlen + W - 1) / W) * W;
-        const int strips = qp / W;
-        int G = req_g > 0 ? std::min(req_g, kWave) : std::min(strips, kWave);
-        const int passes = (strips + G - 1) / G;
-        const int gpw = kWave / G;
-        const double lane_use = (double)strips / ((double)passes * G) * (double)(gpw * G) / kWave;
-        const double eff = lane_use * (9.0 * W) / (9.5 * W + 13.0) * ((double)qlen / qp);
-        if (eff > best) {
-            best = eff;
-            bw = W;
-            bg = G;
-        }
-    }
-    *W_out = bw;
-    *G_out = bg;
-}
+namespace {
 
-int swh_search(swh_ctx* c, swh_db* db, const uint8_t* query, int qlen,
-               const swh_search_opts* opts, int32_t* scores, swh_stats* stats) {
+struct Plan {
+    int qlen;
+    int qpad_l, qpad_g, qpad_r, pstride, G;
+    size_t prof_off;   // int16 elements
+};
+
+}  // namespace
+
+// All queries of a call in one launch per kernel: group scan (pair blocks),
+// lane scan (lane blocks), then one int32 rescore launch over every query's
+// overflow list.  scores_host gets nq rows of nseq (may be NULL for no copy).
+static int run_queries(swh_ctx* c, swh_db* db, const uint8_t* const* qs, const int* qlens,
+                       int nq, const swh_search_opts& o, int32_t* scores_host,
+                       swh_stats* stats) {
     const double t_start = now_ms();
-    if (!c || !db || !query || (!scores && db->nseq > 0))
-        return fail(SWH_ERR_ARG, "NULL argument");
-    if (db->ctx != c) return fail(SWH_ERR_ARG, "database belongs to another context");
-    if (qlen <= 0) return fail(SWH_ERR_ARG, "query length %d", qlen);
-    for (int j = 0; j < qlen; ++j)
-        if (query[j] > 23) return fail(SWH_ERR_ARG, "query code %d at %d > 23", query[j], j);
-    swh_search_opts o{};
-    if (opts) o = *opts;
+    swh_stats st{};
+    for (int i = 0; i < nq; ++i) {
+        if (!qs[i]) return fail(SWH_ERR_ARG, "query %d is NULL", i);
+        if (qlens[i] <= 0) return fail(SWH_ERR_ARG, "query %d length %d", i, qlens[i]);
+        for (int j = 0; j < qlens[i]; ++j)
+            if (qs[i][j] > 23)
+                return fail(SWH_ERR_ARG, "query %d code %d at %d > 23", i, qs[i][j], j);
+        st.cells += db->residues * (uint64_t)qlens[i];
+        st.cells_group += db->residues_pair * (uint64_t)qlens[i];
+    }
     if (o.group_width && o.group_width != 8 && o.group_width != 16)
         return fail(SWH_ERR_ARG, "group_width must be 8 or 16");
     if (o.group_lanes < 0 || o.group_lanes > kWave)
         return fail(SWH_ERR_ARG, "group_lanes must be in 1..64");
+    if (o.strip_width && o.strip_width != 16 && o.strip_width != 24 && o.strip_width != 32)
+        return fail(SWH_ERR_ARG, "strip_width must be 16, 24 or 32");
     HIP_TRY(hipSetDevice(c->device));
-    swh_stats st{};
-    st.cells = db->residues * (uint64_t)qlen;
-    st.cells_group = db->residues_pair * (uint64_t)qlen;
-    if (db->nseq == 0) {
+    if (db->nseq == 0 || nq == 0) {
         st.total_ms = now_ms() - t_start;
         if (stats) *stats = st;
         return SWH_OK;
     }
-    const int Wl = pick_width(qlen, o.strip_width);
-    const int qpad_l = ((qlen + Wl - 1) / Wl) * Wl;
-    int Wg = 16, G = 64;
-    pick_group(qlen, o.group_width, o.group_lanes, &Wg, &G);
-    const int qpad_g = ((qlen + Wg - 1) / Wg) * Wg;
-    const int qpad_r = ((qlen + 7) / 8) * 8;
-    const int pstride = std::max(qpad_l, qpad_g);
-    {
-        const int passes = (qpad_g / Wg + G - 1) / G;
-        const int gpw = kWave / G;
-        (void)gpw;
-        st.padded_cells = db->padded_lane * (uint64_t)qpad_l +
-                          db->padded_pair * (uint64_t)passes * G * Wg;
+    // strip widths shared by the batch: least padded work
+    int Wl = o.strip_width;
+    if (!Wl) {
+        uint64_t best = UINT64_MAX;
+        for (int w : {16, 24, 32}) {
+            uint64_t cols = 0;
+            for (int i = 0; i < nq; ++i) cols += (uint64_t)((qlens[i] + w - 1) / w) * w;
+            if (cols < best) {
+                best = cols;
+                Wl = w;
+            }
+        }
+    }
+    int Wg = o.group_width;
+    if (!Wg) {
+        double best = -1;
+        for (int w : {16, 8}) {
+            double e = 0;
+            int g;
+            for (int i = 0; i < nq; ++i) e += group_eff(qlens[i], w, o.group_lanes, &g) * qlens[i];
+            if (e > best) {
+                best = e;
+                Wg = w;
+            }
+        }
+    }
+    std::vector<Plan> plan(nq);
+    size_t prof_elems = 0;
+    int max_qpad_l = 0, max_pass_g = 0, max_gpw = 1;
+    bool lane_border = false, group_border = false;
+    for (int i = 0; i < nq; ++i) {
+        Plan& P = plan[i];
+        P.qlen = qlens[i];
+        P.qpad_l = ((P.qlen + Wl - 1) / Wl) * Wl;
+        P.qpad_g = ((P.qlen + Wg - 1) / Wg) * Wg;
+        P.qpad_r = ((P.qlen + 7) / 8) * 8;
+        group_eff(P.qlen, Wg, o.group_lanes, &P.G);
+        P.pstride = std::max({P.qpad_l, P.qpad_g, P.qpad_r});
+        P.prof_off = prof_elems;
+        prof_elems += (size_t)kProfRows * P.pstride;
+        max_qpad_l = std::max(max_qpad_l, P.qpad_l);
+        max_pass_g = std::max(max_pass_g, std::min(P.G * Wg, P.qpad_g));
+        max_gpw = std::max(max_gpw, kWave / P.G);
+        lane_border |= P.qpad_l > Wl;
+        group_border |= P.qpad_g > P.G * Wg;
+        const int passes = (P.qpad_g / Wg + P.G - 1) / P.G;
+        st.padded_cells += db->padded_lane * (uint64_t)P.qpad_l +
+                           db->padded_pair * (uint64_t)passes * P.G * Wg;
     }
 
-    // query profile [24][pstride] int16, zero beyond qlen
-    const size_t prof_bytes = (size_t)kProfRows * pstride * sizeof(int16_t);
-    HIP_TRY(c->h_prof.reserve(prof_bytes));
-    HIP_TRY(c->prof.reserve(prof_bytes));
+    // profiles [24][pstride] int16 per query, zero beyond qlen
+    HIP_TRY(c->h_prof.reserve(prof_elems * sizeof(int16_t)));
+    HIP_TRY(c->prof.reserve(prof_elems * sizeof(int16_t)));
     int16_t* hp = (int16_t*)c->h_prof.p;
-    for (int r = 0; r < kProfRows; ++r)
-        for (int j = 0; j < pstride; ++j)
-            hp[(size_t)r * pstride + j] = j < qlen ? (int16_t)c->matrix[query[j] * 32 + r] : 0;
-    HIP_TRY(hipMemcpyAsync(c->prof.p, hp, prof_bytes, hipMemcpyHostToDevice, c->stream));
-
-    const size_t sc_bytes = db->nseq * sizeof(int32_t);
-    HIP_TRY(c->scores.reserve(sc_bytes));
-    HIP_TRY(c->ovf.reserve(db->nseq * sizeof(uint32_t)));
-    uint32_t* ctr = (uint32_t*)c->counters.p;
-    HIP_TRY(launch_reset(ctr, 64, c->stream));
+    for (int i = 0; i < nq; ++i) {
+        const Plan& P = plan[i];
+        int16_t* dst = hp + P.prof_off;
+        for (int r = 0; r < kProfRows; ++r)
+            for (int j = 0; j < P.pstride; ++j)
+                dst[(size_t)r * P.pstride + j] =
+                    j < P.qlen ? (int16_t)c->matrix[qs[i][j] * 32 + r] : 0;
+    }
+    HIP_TRY(hipMemcpyAsync(c->prof.p, hp, prof_elems * sizeof(int16_t), hipMemcpyHostToDevice,
+                           c->stream));
+    const size_t nseq = db->nseq;
+    HIP_TRY(c->scores.reserve((size_t)nq * nseq * sizeof(int32_t)));
+    HIP_TRY(c->ovf.reserve((size_t)nq * nseq * sizeof(uint32_t)));
+    const size_t n_ctr = 16 + (size_t)nq;
+    HIP_TRY(c->counters.reserve(n_ctr * sizeof(uint32_t)));
+    uint32_t* ctr = (uint32_t*)c->counters.p;   // [0] lane, [1] group, [2] rescore; [16+q] overflow count
+    HIP_TRY(launch_reset(ctr, (int)n_ctr, c->stream));
     const int exact_thr = 65535 - c->max_score;
     const int thr = (o.overflow_threshold > 0) ? std::min(o.overflow_threshold, exact_thr)
                                                : exact_thr;
     const int gap_oe = c->gap_open + c->gap_extend;
-
-    // border scratch: the two scans run one after the other on the stream
     const int n_lane = db->n_blocks - db->n_pairs;
-    const int segw_l = std::min(qpad_l, kSegCols);
-    const int ps_l = segw_l + kProfPad;
+
+    // query tables: lane, group, rescore
+    HIP_TRY(c->h_desc.reserve(3 * nq * sizeof(QueryDesc)));
+    HIP_TRY(c->desc.reserve(3 * nq * sizeof(QueryDesc)));
+    QueryDesc* hd = (QueryDesc*)c->h_desc.p;
+    int items_l = 0, items_g = 0;
+    for (int i = 0; i < nq; ++i) {
+        QueryDesc base{};
+        base.prof = (const int16_t*)c->prof.p + plan[i].prof_off;
+        base.scores = (int32_t*)c->scores.p + (size_t)i * nseq;
+        base.ovf_list = (uint32_t*)c->ovf.p + (size_t)i * nseq;
+        base.ovf_count = ctr + 16 + i;
+        base.qlen = plan[i].qlen;
+        base.pstride = plan[i].pstride;
+        base.ovf_threshold = thr;
+        QueryDesc& L = hd[i];
+        L = base;
+        L.qpad = plan[i].qpad_l;
+        L.group = 1;
+        L.item_begin = items_l;
+        L.n_items = (n_lane + kWavesPerWG - 1) / kWavesPerWG;
+        items_l += L.n_items;
+        QueryDesc& Gd = hd[nq + i];
+        Gd = base;
+        Gd.qpad = plan[i].qpad_g;
+        Gd.group = plan[i].G;
+        Gd.item_begin = items_g;
+        const int per_item = kWavesPerWG * (kWave / plan[i].G);
+        Gd.n_items = (db->n_pairs + per_item - 1) / per_item;
+        items_g += Gd.n_items;
+        QueryDesc& Rd = hd[2 * nq + i];
+        Rd = base;
+        Rd.qpad = plan[i].qpad_r;
+        Rd.group = 1;
+    }
+    HIP_TRY(hipMemcpyAsync(c->desc.p, hd, 2 * nq * sizeof(QueryDesc), hipMemcpyHostToDevice,
+                           c->stream));
+    const QueryDesc* d_lane = (const QueryDesc*)c->desc.p;
+    const QueryDesc* d_group = d_lane + nq;
+    const QueryDesc* d_res = d_lane + 2 * nq;
+
+    // launch shapes and strip-border scratch (the scans run one after the other)
+    const int ps_l = std::min(max_qpad_l, kSegCols) + kProfPad;
     const size_t lds_l = (size_t)kProfRows * ps_l * sizeof(int16_t);
-    const int items_l = (n_lane + kWavesPerWG - 1) / kWavesPerWG;
+    const uint64_t stride_l = lane_border ? (uint64_t)db->max_rows_lane * kWave : 0;
+    const uint64_t per_wg_l = stride_l * sizeof(uint2) * kWavesPerWG;
     int grid_l = c->num_cus * lane_scan_max_wg_per_cu(Wl, lds_l);
-    const uint64_t stride_l = (uint64_t)db->max_rows_lane * kWave;
-    const uint64_t per_wg_l = qpad_l > Wl ? stride_l * sizeof(uint2) * kWavesPerWG : 0;
-    if (per_wg_l) grid_l = (int)std::min<uint64_t>(grid_l, std::max<uint64_t>(1, c->border_budget / per_wg_l));
+    if (per_wg_l)
+        grid_l = (int)std::min<uint64_t>(grid_l, std::max<uint64_t>(1, c->border_budget / per_wg_l));
     grid_l = std::max(1, std::min(grid_l, items_l));
 
-    const int gpw = kWave / G;
-    const int pass_cols = G * Wg;
-    const int ps_g = std::min(pass_cols, qpad_g) + kProfPad;
+    const int ps_g = max_pass_g + kProfPad;
     const size_t lds_g = (size_t)kProfRows * ps_g * sizeof(int16_t);
-    const int items_g = (db->n_pairs + kWavesPerWG * gpw - 1) / (kWavesPerWG * gpw);
+    const uint64_t stride_g = group_border ? (uint64_t)max_gpw * db->max_rows_pair : 0;
+    const uint64_t per_wg_g = stride_g * sizeof(uint2) * kWavesPerWG;
     int grid_g = c->num_cus * group_scan_max_wg_per_cu(Wg, lds_g);
-    const uint64_t stride_g = (uint64_t)gpw * db->max_rows_pair;
-    const uint64_t per_wg_g = qpad_g > pass_cols ? stride_g * sizeof(uint2) * kWavesPerWG : 0;
-    if (per_wg_g) grid_g = (int)std::min<uint64_t>(grid_g, std::max<uint64_t>(1, c->border_budget / per_wg_g));
+    if (per_wg_g)
+        grid_g = (int)std::min<uint64_t>(grid_g, std::max<uint64_t>(1, c->border_budget / per_wg_g));
     grid_g = std::max(1, std::min(grid_g, items_g));
     HIP_TRY(c->border.reserve(std::max<uint64_t>(
         {16, per_wg_l * (uint64_t)grid_l, per_wg_g * (uint64_t)grid_g})));
 
+    ScanArgs a{};
+    a.words = (const uint32_t*)db->words.p;
+    a.blocks = (const BlockDesc*)db->d_blocks.p;
+    a.perm = (const uint32_t*)db->perm.p;
+    a.border = (uint2*)c->border.p;
+    a.n_queries = nq;
+    a.gap_oe = gap_oe;
+    a.gap_e = c->gap_extend;
+
     HIP_TRY(hipEventRecord(c->ev[0], c->stream));
-    if (db->n_pairs > 0) {
-        GroupArgs g{};
-        g.words = (const uint32_t*)db->words.p;
-        g.blocks = (const BlockDesc*)db->d_blocks.p;
-        g.perm = (const uint32_t*)db->perm.p;
-        g.prof = (const int16_t*)c->prof.p;
-        g.scores = (int32_t*)c->scores.p;
-        g.ovf_list = (uint32_t*)c->ovf.p;
-        g.ovf_count = ctr + 1;
-        g.work_counter = ctr + 3;
-        g.border = (uint2*)c->border.p;
+    if (db->n_pairs > 0 && items_g > 0) {
+        ScanArgs g = a;
+        g.queries = d_group;
+        g.work_counter = ctr + 1;
         g.border_stride = stride_g;
         g.border_rows = db->max_rows_pair;
+        g.n_items = items_g;
         g.blk_begin = 0;
         g.blk_end = db->n_pairs;
-        g.group = G;
-        g.n_items = items_g;
-        g.qlen = qlen;
-        g.qpad = qpad_g;
         g.ps = ps_g;
-        g.pstride = pstride;
-        g.gap_oe = gap_oe;
-        g.gap_e = c->gap_extend;
-        g.ovf_threshold = thr;
         HIP_TRY(launch_group_scan(Wg, g, grid_g, lds_g, c->stream));
         st.n_launches += 1;
     }
     HIP_TRY(hipEventRecord(c->ev[4], c->stream));
-    if (n_lane > 0) {
-        ScanArgs a{};
-        a.words = (const uint32_t*)db->words.p;
-        a.blocks = (const BlockDesc*)db->d_blocks.p;
-        a.perm = (const uint32_t*)db->perm.p;
-        a.prof = (const int16_t*)c->prof.p;
-        a.scores = (int32_t*)c->scores.p;
-        a.ovf_list = (uint32_t*)c->ovf.p;
-        a.ovf_count = ctr + 1;
-        a.work_counter = ctr + 0;
-        a.border = (uint2*)c->border.p;
-        a.border_stride = stride_l;
-        a.blk_begin = db->n_pairs;
-        a.blk_end = db->n_blocks;
-        a.n_items = items_l;
-        a.qlen = qlen;
-        a.qpad = qpad_l;
-        a.ps = ps_l;
-        a.pstride = pstride;
-        a.gap_oe = gap_oe;
-        a.gap_e = c->gap_extend;
-        a.ovf_threshold = thr;
-        HIP_TRY(launch_lane_scan(Wl, a, grid_l, lds_l, c->stream));
+    if (n_lane > 0 && items_l > 0) {
+        ScanArgs l = a;
+        l.queries = d_lane;
+        l.work_counter = ctr + 0;
+        l.border_stride = stride_l;
+        l.n_items = items_l;
+        l.blk_begin = db->n_pairs;
+        l.blk_end = db->n_blocks;
+        l.ps = ps_l;
+        HIP_TRY(launch_lane_scan(Wl, l, grid_l, lds_l, c->stream));
         st.n_launches += 1;
     }
     HIP_TRY(hipEventRecord(c->ev[1], c->stream));
 
-    // int32 rescore of the overflow list (the kernel reads the count itself)
-    uint32_t* h_cnt = (uint32_t*)c->h_scores.p;
-    HIP_TRY(c->h_scores.reserve(std::max<size_t>(sc_bytes, 64)));
-    h_cnt = (uint32_t*)c->h_scores.p;
-    HIP_TRY(hipMemcpyAsync(h_cnt, ctr + 1, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    // overflow counts -> rescore work items
+    HIP_TRY(c->h_ctr.reserve(n_ctr * sizeof(uint32_t)));
+    uint32_t* hc = (uint32_t*)c->h_ctr.p;
+    HIP_TRY(hipMemcpyAsync(hc, ctr, n_ctr * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
-    const uint32_t n_ovf = *h_cnt;
-    st.n_overflow = n_ovf;
-    if (n_ovf > 0) {
+    int items_r = 0;
+    for (int i = 0; i < nq; ++i) {
+        QueryDesc& Rd = hd[2 * nq + i];
+        Rd.item_begin = items_r;
+        Rd.n_items = (int)((hc[16 + i] + 255) / 256);
+        items_r += Rd.n_items;
+        st.n_overflow += hc[16 + i];
+    }
+    if (items_r > 0) {
+        HIP_TRY(hipMemcpyAsync((void*)d_res, hd + 2 * nq, nq * sizeof(QueryDesc),
+                               hipMemcpyHostToDevice, c->stream));
         RescoreArgs r{};
-        r.words = (const uint32_t*)db->words.p;
-        r.blocks = (const BlockDesc*)db->d_blocks.p;
-        r.perm = (const uint32_t*)db->perm.p;
+        r.words = a.words;
+        r.blocks = a.blocks;
+        r.perm = a.perm;
         r.sorted_len = (const uint32_t*)db->sorted_len.p;
         r.sorted_blk = (const uint32_t*)db->sorted_blk.p;
-        r.prof = (const int16_t*)c->prof.p;
-        r.scores = (int32_t*)c->scores.p;
-        r.ovf_list = (const uint32_t*)c->ovf.p;
-        r.ovf_count = ctr + 1;
+        r.queries = d_res;
         r.work_counter = ctr + 2;
+        r.n_queries = nq;
+        r.n_items = items_r;
         r.n_lane_blocks_first = db->n_pairs;
-        r.qpad = qpad_r;
-        r.ps = std::min(qpad_r, kSegCols) + kProfPad;
-        r.pstride = pstride;
+        int max_qpad_r = 0;
+        for (const Plan& P : plan) max_qpad_r = std::max(max_qpad_r, P.qpad_r);
+        r.ps = std::min(max_qpad_r, kSegCols) + kProfPad;
         r.gap_oe = gap_oe;
         r.gap_e = c->gap_extend;
         const size_t lds_r = (size_t)kProfRows * r.ps * sizeof(int16_t);
         const uint32_t lmax = std::max(db->max_rows_lane, db->max_rows_pair);
         const uint64_t rstride = (uint64_t)lmax * kWave;
         const uint64_t r_per_wg = rstride * sizeof(uint2) * kWavesPerWG;
-        int rgrid = (int)std::min<uint64_t>((n_ovf + 255) / 256,
+        int rgrid = (int)std::min<uint64_t>((uint64_t)items_r,
                                             (uint64_t)c->num_cus * rescore_max_wg_per_cu(lds_r));
         rgrid = (int)std::max<uint64_t>(1, std::min<uint64_t>(rgrid, c->border_budget / r_per_wg));
         HIP_TRY(c->border.reserve(r_per_wg * rgrid));
@@ -581,10 +643,21 @@ int swh_search(swh_ctx* c, swh_db* db, const uint8_t* query, int qlen,
         HIP_TRY(hipEventRecord(c->ev[3], c->stream));
         st.n_launches += 1;
     }
-    HIP_TRY(hipMemcpyAsync(c->h_scores.p, c->scores.p, sc_bytes, hipMemcpyDeviceToHost,
-                           c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    std::memcpy(scores, c->h_scores.p, sc_bytes);
+    if (scores_host) {
+        const size_t bytes = (size_t)nq * nseq * sizeof(int32_t);
+        if (bytes <= (size_t(64) << 20)) {
+            HIP_TRY(c->h_scores.reserve(bytes));
+            HIP_TRY(hipMemcpyAsync(c->h_scores.p, c->scores.p, bytes, hipMemcpyDeviceToHost,
+                                   c->stream));
+            HIP_TRY(hipStreamSynchronize(c->stream));
+            std::memcpy(scores_host, c->h_scores.p, bytes);
+        } else {
+            HIP_TRY(hipStreamSynchronize(c->stream));
+            HIP_TRY(hipMemcpy(scores_host, c->scores.p, bytes, hipMemcpyDeviceToHost));
+        }
+    } else {
+        HIP_TRY(hipStreamSynchronize(c->stream));
+    }
     float ms = 0;
     HIP_TRY(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
     st.kernel_ms = ms;
@@ -592,7 +665,7 @@ int swh_search(swh_ctx* c, swh_db* db, const uint8_t* query, int qlen,
     st.group_ms = ms;
     HIP_TRY(hipEventElapsedTime(&ms, c->ev[4], c->ev[1]));
     st.lane_ms = ms;
-    if (n_ovf > 0) {
+    if (items_r > 0) {
         HIP_TRY(hipEventElapsedTime(&ms, c->ev[2], c->ev[3]));
         st.rescore_ms = ms;
     }
@@ -601,29 +674,25 @@ int swh_search(swh_ctx* c, swh_db* db, const uint8_t* query, int qlen,
     return SWH_OK;
 }
 
+int swh_search(swh_ctx* c, swh_db* db, const uint8_t* query, int qlen,
+               const swh_search_opts* opts, int32_t* scores, swh_stats* stats) {
+    if (!c || !db || !query || (!scores && db->nseq > 0))
+        return fail(SWH_ERR_ARG, "NULL argument");
+    if (db->ctx != c) return fail(SWH_ERR_ARG, "database belongs to another context");
+    swh_search_opts o{};
+    if (opts) o = *opts;
+    const uint8_t* qs[1] = {query};
+    return run_queries(c, db, qs, &qlen, 1, o, scores, stats);
+}
+
 int swh_search_batch(swh_ctx* c, swh_db* db, const uint8_t* const* queries, const int* qlens,
                      int nq, const swh_search_opts* opts, int32_t* scores, swh_stats* stats) {
-    if (!c || !db || (nq > 0 && (!queries || !qlens || !scores)))
+    if (!c || !db || nq < 0 || (nq > 0 && (!queries || !qlens || (!scores && db->nseq > 0))))
         return fail(SWH_ERR_ARG, "NULL argument");
-    swh_stats total{};
-    const double t0 = now_ms();
-    for (int i = 0; i < nq; ++i) {
-        swh_stats st{};
-        int rc = swh_search(c, db, queries[i], qlens[i], opts, scores + (size_t)i * db->nseq, &st);
-        if (rc) return rc;
-        total.kernel_ms += st.kernel_ms;
-        total.group_ms += st.group_ms;
-        total.lane_ms += st.lane_ms;
-        total.cells_group += st.cells_group;
-        total.rescore_ms += st.rescore_ms;
-        total.cells += st.cells;
-        total.padded_cells += st.padded_cells;
-        total.n_overflow += st.n_overflow;
-        total.n_launches += st.n_launches;
-    }
-    total.total_ms = now_ms() - t0;
-    if (stats) *stats = total;
-    return SWH_OK;
+    if (db->ctx != c) return fail(SWH_ERR_ARG, "database belongs to another context");
+    swh_search_opts o{};
+    if (opts) o = *opts;
+    return run_queries(c, db, queries, qlens, nq, o, scores, stats);
 }
 
 int swh_topk(const int32_t* scores, uint64_t n, int k, int32_t* out_score, int64_t* out_index) {

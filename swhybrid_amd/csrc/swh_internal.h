@@ -26,48 +26,46 @@ struct BlockDesc {
     uint32_t first;      // sorted position of the block's first sequence
 };
 
+// One query of a launch, as one kernel sees it.  Kernels take a table of
+// these, so one persistent launch serves many queries through one device work
+// queue; the work items of query q are [item_begin, item_begin + n_items).
+struct QueryDesc {
+    const int16_t* prof;     // profile [24][pstride] int16, zero beyond qlen
+    int32_t* scores;         // caller order, one per database sequence
+    uint32_t* ovf_list;      // sorted positions sent to the int32 rescore
+    uint32_t* ovf_count;
+    int qlen;
+    int qpad;                // qlen rounded up to this kernel's strip width
+    int pstride;             // profile row stride (int16), >= qpad
+    int group;               // group kernel: lanes per group (1..64)
+    int item_begin;
+    int n_items;
+    int ovf_threshold;       // score >= threshold -> int32 rescore
+    int pad_;
+};
+
 struct ScanArgs {
     const uint32_t* words;     // packed residues
     const BlockDesc* blocks;   // block table
     const uint32_t* perm;      // sorted position -> caller index (kNoSeq for fillers)
-    const int16_t* prof;       // query profile [24][qpad], zero beyond qlen
-    int32_t* scores;           // caller order
-    uint32_t* ovf_list;        // sorted positions to rescore
-    uint32_t* ovf_count;
+    const QueryDesc* queries;
     uint32_t* work_counter;
-    uint2* border;             // per wave slot: rows * 64 uint2 (H pair, F pair)
+    uint2* border;             // per wave slot (see the kernels)
     uint64_t border_stride;    // uint2 elements per wave slot
+    uint32_t border_rows;      // group kernel: rows reserved per group
+    int n_queries;
+    int n_items;               // all queries
     int blk_begin, blk_end;    // block range of this launch
-    int n_items;               // work items (4 blocks each)
-    int qlen, qpad, ps;        // query length, padded length, LDS row stride (int16)
-    int pstride;               // row stride of prof (int16), >= qpad
+    int ps;                    // LDS profile row stride (int16)
     int gap_oe, gap_e;
-    int ovf_threshold;         // score >= threshold -> int32 rescore
 };
 
-// Group mode: G lanes cooperate on one sequence pair.  Lane g of a group holds
-// query columns [pass*G*W + g*W, +W) and works on row t-g at step t; the H/F
-// of its last column reach lane g+1 one step later through a wave-wide DPP
-// shift.  A wave holds floor(64/G) groups; a workgroup item is 4 waves.
-struct GroupArgs {
-    const uint32_t* words;
-    const BlockDesc* blocks;   // pair blocks (2 sequences each)
-    const uint32_t* perm;
-    const int16_t* prof;
-    int32_t* scores;
-    uint32_t* ovf_list;
-    uint32_t* ovf_count;
-    uint32_t* work_counter;
-    uint2* border;             // per wave slot: groups_per_wave * border_rows uint2
-    uint64_t border_stride;    // uint2 per wave slot
-    uint32_t border_rows;      // rows reserved per group (>= longest pair block)
-    int blk_begin, blk_end;    // pair-block range
-    int group;                 // G, lanes per group (1..64)
-    int n_items;
-    int qlen, qpad, ps, pstride;
-    int gap_oe, gap_e;
-    int ovf_threshold;
-};
+// Lane kernel: a work item is 4 consecutive lane blocks (one per wave) of one
+// query.  Group kernel: G lanes cooperate on one sequence pair; lane g of a
+// group holds query columns [pass*G*W + g*W, +W) and works on row t-g at step
+// t, the H/F of its last column reaching lane g+1 one step later through a
+// wave-wide DPP shift; a wave holds floor(64/G) groups and a work item is 4
+// waves of pairs.  Rescore: a work item is 256 overflow-list entries.
 
 struct RescoreArgs {
     const uint32_t* words;
@@ -75,15 +73,14 @@ struct RescoreArgs {
     const uint32_t* perm;
     const uint32_t* sorted_len;   // real length per sorted position
     const uint32_t* sorted_blk;   // block of each sorted position
-    const int16_t* prof;
-    int32_t* scores;
-    const uint32_t* ovf_list;
-    const uint32_t* ovf_count;
+    const QueryDesc* queries;     // qpad = qlen rounded to 8; n_items from the overflow counts
     uint32_t* work_counter;
     uint2* border;
     uint64_t border_stride;
+    int n_queries;
+    int n_items;
     int n_lane_blocks_first;      // index of the first lane-mode block
-    int qpad, ps, pstride;
+    int ps;
     int gap_oe, gap_e;
 };
 
@@ -91,7 +88,7 @@ struct RescoreArgs {
 hipError_t launch_reset(uint32_t* p, int n, hipStream_t s);
 hipError_t launch_lane_scan(int width, const ScanArgs& a, int grid, size_t lds_bytes,
                             hipStream_t s);
-hipError_t launch_group_scan(int width, const GroupArgs& a, int grid, size_t lds_bytes,
+hipError_t launch_group_scan(int width, const ScanArgs& a, int grid, size_t lds_bytes,
                              hipStream_t s);
 int group_scan_max_wg_per_cu(int width, size_t lds_bytes);
 hipError_t launch_rescore(const RescoreArgs& a, int grid, size_t lds_bytes, hipStream_t s);

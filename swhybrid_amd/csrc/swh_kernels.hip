@@ -162,60 +162,83 @@ __device__ __forceinline__ void lane_strip(const uint2* __restrict__ wds, int qu
     S = vmax(S, reduce_max<W, s2>(M));
 }
 
-__device__ __forceinline__ void emit(const ScanArgs& A, uint32_t s, int sc) {
-    const uint32_t idx = A.perm[s];
+__device__ __forceinline__ void emit(const QueryDesc& Qd, const uint32_t* perm, uint32_t s,
+                                     int sc) {
+    const uint32_t idx = perm[s];
     if (idx == kNoSeq) return;
-    if (sc >= A.ovf_threshold) {
-        const uint32_t slot = atomicAdd(A.ovf_count, 1u);
-        A.ovf_list[slot] = s;
-        A.scores[idx] = -1;
+    if (sc >= Qd.ovf_threshold) {
+        const uint32_t slot = atomicAdd(Qd.ovf_count, 1u);
+        Qd.ovf_list[slot] = s;
+        Qd.scores[idx] = -1;
     } else {
-        A.scores[idx] = sc;
+        Qd.scores[idx] = sc;
     }
+}
+
+// Thread 0 claims the next work item and finds its query (the last query whose
+// first item is <= the claimed one); the workgroup reads both from LDS.
+__device__ __forceinline__ bool next_item(uint32_t* counter, const QueryDesc* qs, int nq,
+                                          int n_items, int* s_buf, int& item, int& q) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int it = (int)atomicAdd(counter, 1u);
+        int lo = 0, hi = nq - 1;
+        if (it < n_items) {
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (qs[mid].item_begin <= it) lo = mid;
+                else hi = mid - 1;
+            }
+        }
+        s_buf[0] = it;
+        s_buf[1] = lo;
+    }
+    __syncthreads();
+    item = s_buf[0];
+    q = s_buf[1];
+    return item < n_items;
 }
 
 template <int W>
 __global__ __launch_bounds__(256) void k_lane_scan(ScanArgs A) {
     extern __shared__ __align__(16) short lds[];
-    __shared__ int s_item;
+    __shared__ int s_buf[2];
     const int wave = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
     uint2* bord = A.border + (size_t)(blockIdx.x * kWavesPerWG + wave) * A.border_stride;
     const s2 Q = {(short)A.gap_oe, (short)A.gap_oe};
     const s2 R = {(short)A.gap_e, (short)A.gap_e};
-    const int nseg = (A.qpad + kSegCols - 1) / kSegCols;
-    int loaded = -1;
-    for (;;) {
-        __syncthreads();
-        if (threadIdx.x == 0) s_item = (int)atomicAdd(A.work_counter, 1u);
-        __syncthreads();
-        const int item = s_item;
-        if (item >= A.n_items) break;
-        const int blk = A.blk_begin + item * kWavesPerWG + wave;
+    int loaded_q = -1, loaded_seg = -1;
+    int item, q;
+    while (next_item(A.work_counter, A.queries, A.n_queries, A.n_items, s_buf, item, q)) {
+        const QueryDesc Qd = A.queries[q];
+        const int blk = A.blk_begin + (item - Qd.item_begin) * kWavesPerWG + wave;
         const bool active = blk < A.blk_end;
         BlockDesc bd = {0, 0, 0};
         if (active) bd = A.blocks[blk];
         const uint2* wds = reinterpret_cast<const uint2*>(A.words + bd.word_off);
         const int quads = (int)(bd.rows >> 2);
+        const int nseg = (Qd.qpad + kSegCols - 1) / kSegCols;
         s2 S = as_s2(kZero16x2);
         for (int seg = 0; seg < nseg; ++seg) {
             const int seg0 = seg * kSegCols;
-            const int segw = min(kSegCols, A.qpad - seg0);
-            if (seg != loaded) {
+            const int segw = min(kSegCols, Qd.qpad - seg0);
+            if (q != loaded_q || seg != loaded_seg) {
                 __syncthreads();
-                stage_profile(lds, A.prof, A.pstride, A.ps, seg0, segw);
+                stage_profile(lds, Qd.prof, Qd.pstride, A.ps, seg0, segw);
                 __syncthreads();
-                loaded = seg;
+                loaded_q = q;
+                loaded_seg = seg;
             }
             if (active) {
                 for (int c0 = seg0; c0 < seg0 + segw; c0 += W)
-                    lane_strip<W>(wds, quads, bord, lds, A.ps, c0 - seg0, c0 > 0, c0 + W < A.qpad,
-                                  Q, R, S, lane);
+                    lane_strip<W>(wds, quads, bord, lds, A.ps, c0 - seg0, c0 > 0,
+                                  c0 + W < Qd.qpad, Q, R, S, lane);
             }
         }
         if (active) {
-            emit(A, bd.first + lane, (int)S.x + 32768);
-            emit(A, bd.first + kWave + lane, (int)S.y + 32768);
+            emit(Qd, A.perm, bd.first + lane, (int)S.x + 32768);
+            emit(Qd, A.perm, bd.first + kWave + lane, (int)S.y + 32768);
         }
     }
 }
@@ -233,31 +256,26 @@ __device__ __forceinline__ uint32_t wave_shr1(uint32_t v) {
 }
 
 template <int W>
-__global__ __launch_bounds__(256) void k_group_scan(GroupArgs A) {
+__global__ __launch_bounds__(256) void k_group_scan(ScanArgs A) {
     extern __shared__ __align__(16) short lds[];
-    __shared__ int s_item;
+    __shared__ int s_buf[2];
     __shared__ uint32_t s_red[kWavesPerWG][kWave];
     const int wave = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
-    const int G = A.group;
-    const int gpw = kWave / G;
-    const int gi = lane / G;
-    const int gl = lane - gi * G;
-    const bool in_group = gi < gpw;
-    uint2* bord = A.border + (size_t)(blockIdx.x * kWavesPerWG + wave) * A.border_stride +
-                  (size_t)(in_group ? gi : 0) * A.border_rows;
+    uint2* slot = A.border + (size_t)(blockIdx.x * kWavesPerWG + wave) * A.border_stride;
     const s2 Q = {(short)A.gap_oe, (short)A.gap_oe};
     const s2 R = {(short)A.gap_e, (short)A.gap_e};
     const s2 Z = as_s2(kZero16x2);
-    const int pass_cols = G * W;
-    const int npass = (A.qpad + pass_cols - 1) / pass_cols;
-    for (;;) {
-        __syncthreads();
-        if (threadIdx.x == 0) s_item = (int)atomicAdd(A.work_counter, 1u);
-        __syncthreads();
-        const int item = s_item;
-        if (item >= A.n_items) break;
-        const int p = A.blk_begin + (item * kWavesPerWG + wave) * gpw + gi;
+    int item, q;
+    while (next_item(A.work_counter, A.queries, A.n_queries, A.n_items, s_buf, item, q)) {
+        const QueryDesc Qd = A.queries[q];
+        const int G = Qd.group;
+        const int gpw = kWave / G;
+        const int gi = lane / G;
+        const int gl = lane - gi * G;
+        const bool in_group = gi < gpw;
+        uint2* bord = slot + (size_t)(in_group ? gi : 0) * A.border_rows;
+        const int p = A.blk_begin + ((item - Qd.item_begin) * kWavesPerWG + wave) * gpw + gi;
         const bool has_pair = in_group && p < A.blk_end;
         BlockDesc bd = {0, 0, 0};
         if (has_pair) bd = A.blocks[p];
@@ -266,13 +284,15 @@ __global__ __launch_bounds__(256) void k_group_scan(GroupArgs A) {
             rmax = max(rmax, (uint32_t)__shfl_xor((int)rmax, off));
         rmax = __builtin_amdgcn_readfirstlane(rmax);   // equal in every lane: keep loops scalar
         const uint2* pw = reinterpret_cast<const uint2*>(A.words + bd.word_off);
+        const int pass_cols = G * W;
+        const int npass = (Qd.qpad + pass_cols - 1) / pass_cols;
         s2 S = Z;
         for (int pass = 0; pass < npass; ++pass) {
             const int col_base = pass * pass_cols;
-            const int pcols = min(pass_cols, A.qpad - col_base);
+            const int pcols = min(pass_cols, Qd.qpad - col_base);
             const int nl = pcols / W;
             __syncthreads();
-            stage_profile(lds, A.prof, A.pstride, A.ps, col_base, pcols);
+            stage_profile(lds, Qd.prof, Qd.pstride, A.ps, col_base, pcols);
             __syncthreads();
             const bool has_in = pass > 0;
             const bool has_out = pass + 1 < npass;
@@ -289,28 +309,50 @@ __global__ __launch_bounds__(256) void k_group_scan(GroupArgs A) {
             s2 diag0 = Z;
             uint32_t ch = kZero16x2, cf = kZero16x2;   // this lane's last (H, F)
             const int steps = rmax > 0 ? (int)rmax + nl - 1 : 0;
-            for (int t = 0; t < steps; ++t) {
-                uint2 bin = make_uint2(kZero16x2, kZero16x2);
-                if (has_in && gl == 0 && has_pair && t < (int)bd.rows) bin = bord[t];
-                // every lane takes part in the shift (no lane may be masked off here)
-                const uint32_t hx = wave_shr1(ch);
-                const uint32_t fx = wave_shr1(cf);
-                const uint32_t hl_u = gl == 0 ? bin.x : hx;
-                const uint32_t fl_u = gl == 0 ? bin.y : fx;
-                const int row = t - gl;
-                if (on && row >= 0 && row < (int)bd.rows) {
-                    const uint2 w = pw[row >> 2];
-                    const int sh = 8 * (row & 3);
-                    const int a = (w.x >> sh) & 0xff;
-                    const int b = (w.y >> sh) & 0xff;
-                    s2 P[W];
-                    pair_profile<W>(P, lds, A.ps, a, b, jj0);
-                    s2 fout;
-                    row_cells<W, s2>(h, e, hq, M, P, diag0, as_s2(hl_u), as_s2(fl_u), Q, R, fout);
-                    ch = bits(h[W - 1]);
-                    cf = bits(fout);
-                    diag0 = as_s2(hl_u);
-                    if (has_out && gl == nl - 1) bord[row] = make_uint2(ch, cf);
+            // residue words and border entries are loaded 4 steps ahead
+            const bool reads_border = has_in && gl == 0 && has_pair;
+            uint2 wn[4], bn[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int row = u - gl;
+                wn[u] = make_uint2(0u, 0u);
+                bn[u] = make_uint2(kZero16x2, kZero16x2);
+                if (on && row >= 0 && row < (int)bd.rows) wn[u] = pw[row >> 2];
+                if (reads_border && u < (int)bd.rows) bn[u] = bord[u];
+            }
+            for (int t0 = 0; t0 < steps; t0 += 4) {
+                uint2 wc[4], bc[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    wc[u] = wn[u];
+                    bc[u] = bn[u];
+                    const int tn = t0 + 4 + u;
+                    const int row = tn - gl;
+                    if (on && row >= 0 && row < (int)bd.rows) wn[u] = pw[row >> 2];
+                    if (reads_border && tn < (int)bd.rows) bn[u] = bord[tn];
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int row = t0 + u - gl;
+                    // every lane takes part in the shift (no lane may be masked off here)
+                    const uint32_t hx = wave_shr1(ch);
+                    const uint32_t fx = wave_shr1(cf);
+                    const uint32_t hl_u = gl == 0 ? bc[u].x : hx;
+                    const uint32_t fl_u = gl == 0 ? bc[u].y : fx;
+                    if (on && row >= 0 && row < (int)bd.rows) {
+                        const int sh = 8 * (row & 3);
+                        const int a = (wc[u].x >> sh) & 0xff;
+                        const int b = (wc[u].y >> sh) & 0xff;
+                        s2 P[W];
+                        pair_profile<W>(P, lds, A.ps, a, b, jj0);
+                        s2 fout;
+                        row_cells<W, s2>(h, e, hq, M, P, diag0, as_s2(hl_u), as_s2(fl_u), Q, R,
+                                         fout);
+                        ch = bits(h[W - 1]);
+                        cf = bits(fout);
+                        diag0 = as_s2(hl_u);
+                        if (has_out && gl == nl - 1) bord[row] = make_uint2(ch, cf);
+                    }
                 }
             }
             if (on) S = vmax(S, reduce_max<W, s2>(M));
@@ -320,27 +362,8 @@ __global__ __launch_bounds__(256) void k_group_scan(GroupArgs A) {
         if (has_pair && gl == 0) {
             s2 m = S;
             for (int k = 1; k < G; ++k) m = vmax(m, as_s2(s_red[wave][lane + k]));
-            const uint32_t s0 = bd.first;
-            const uint32_t idx0 = A.perm[s0];
-            const uint32_t idx1 = A.perm[s0 + 1];
-            const int sc0 = (int)m.x + 32768;
-            const int sc1 = (int)m.y + 32768;
-            if (idx0 != kNoSeq) {
-                if (sc0 >= A.ovf_threshold) {
-                    A.ovf_list[atomicAdd(A.ovf_count, 1u)] = s0;
-                    A.scores[idx0] = -1;
-                } else {
-                    A.scores[idx0] = sc0;
-                }
-            }
-            if (idx1 != kNoSeq) {
-                if (sc1 >= A.ovf_threshold) {
-                    A.ovf_list[atomicAdd(A.ovf_count, 1u)] = s0 + 1;
-                    A.scores[idx1] = -1;
-                } else {
-                    A.scores[idx1] = sc1;
-                }
-            }
+            emit(Qd, A.perm, bd.first, (int)m.x + 32768);
+            emit(Qd, A.perm, bd.first + 1, (int)m.y + 32768);
         }
     }
 }
@@ -359,27 +382,22 @@ __device__ __forceinline__ void single_profile(int (&P)[W], const short* lds, in
 __global__ __launch_bounds__(256) void k_rescore(RescoreArgs A) {
     constexpr int W = 8;   // divides every padded query length (all are multiples of 8)
     extern __shared__ __align__(16) short lds[];
-    __shared__ int s_item;
+    __shared__ int s_buf[2];
     const int wave = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
     uint2* bord = A.border + (size_t)(blockIdx.x * kWavesPerWG + wave) * A.border_stride;
-    const int n = (int)*A.ovf_count;
-    const int n_items = (n + 255) >> 8;
-    const int nseg = (A.qpad + kSegCols - 1) / kSegCols;
-    int loaded = -1;
-    for (;;) {
-        __syncthreads();
-        if (threadIdx.x == 0) s_item = (int)atomicAdd(A.work_counter, 1u);
-        __syncthreads();
-        const int item = s_item;
-        if (item >= n_items) break;
-        const int entry = item * 256 + threadIdx.x;
+    int loaded_q = -1, loaded_seg = -1;
+    int item, q;
+    while (next_item(A.work_counter, A.queries, A.n_queries, A.n_items, s_buf, item, q)) {
+        const QueryDesc Qd = A.queries[q];
+        const int n = (int)*Qd.ovf_count;
+        const int entry = (item - Qd.item_begin) * 256 + threadIdx.x;
         const bool valid = entry < n;
         uint32_t s = 0, len = 0;
         uint64_t base = 0;
         uint32_t stride = 0;
         if (valid) {
-            s = A.ovf_list[entry];
+            s = Qd.ovf_list[entry];
             len = A.sorted_len[s];
             const uint32_t blk = A.sorted_blk[s];
             const BlockDesc bd = A.blocks[blk];
@@ -397,19 +415,21 @@ __global__ __launch_bounds__(256) void k_rescore(RescoreArgs A) {
         for (int off = 32; off > 0; off >>= 1) lmax = max(lmax, (uint32_t)__shfl_xor((int)lmax, off));
         lmax = __builtin_amdgcn_readfirstlane(lmax);
         const int quads = (int)((lmax + 3) >> 2);
+        const int nseg = (Qd.qpad + kSegCols - 1) / kSegCols;
         int S = kZero32;
         for (int seg = 0; seg < nseg; ++seg) {
             const int seg0 = seg * kSegCols;
-            const int segw = min(kSegCols, A.qpad - seg0);
-            if (seg != loaded) {
+            const int segw = min(kSegCols, Qd.qpad - seg0);
+            if (q != loaded_q || seg != loaded_seg) {
                 __syncthreads();
-                stage_profile(lds, A.prof, A.pstride, A.ps, seg0, segw);
+                stage_profile(lds, Qd.prof, Qd.pstride, A.ps, seg0, segw);
                 __syncthreads();
-                loaded = seg;
+                loaded_q = q;
+                loaded_seg = seg;
             }
             for (int c0 = seg0; c0 < seg0 + segw; c0 += W) {
                 const bool has_in = c0 > 0;
-                const bool has_out = c0 + W < A.qpad;
+                const bool has_out = c0 + W < Qd.qpad;
                 int h[W], e[W], hq[W], M[W];
 #pragma unroll
                 for (int c = 0; c < W; ++c) {
@@ -445,7 +465,7 @@ __global__ __launch_bounds__(256) void k_rescore(RescoreArgs A) {
         }
         if (valid) {
             const uint32_t idx = A.perm[s];
-            if (idx != kNoSeq) A.scores[idx] = (int)((int64_t)S - (int64_t)kZero32);
+            if (idx != kNoSeq) Qd.scores[idx] = (int)((int64_t)S - (int64_t)kZero32);
         }
     }
 }
@@ -477,12 +497,12 @@ hipError_t launch_lane_scan(int width, const ScanArgs& a, int grid, size_t lds, 
 }
 
 template <int W>
-static hipError_t launch_group_w(const GroupArgs& a, int grid, size_t lds, hipStream_t s) {
+static hipError_t launch_group_w(const ScanArgs& a, int grid, size_t lds, hipStream_t s) {
     hipLaunchKernelGGL(k_group_scan<W>, dim3(grid), dim3(256), lds, s, a);
     return hipGetLastError();
 }
 
-hipError_t launch_group_scan(int width, const GroupArgs& a, int grid, size_t lds, hipStream_t s) {
+hipError_t launch_group_scan(int width, const ScanArgs& a, int grid, size_t lds, hipStream_t s) {
     switch (width) {
         case 8: return launch_group_w<8>(a, grid, lds, s);
         case 16: return launch_group_w<16>(a, grid, lds, s);

@@ -176,3 +176,35 @@ def test_group_mode_odd_count_and_overflow(eng):
     got, st = check_db(eng, db, q, min_len=1000, overflow_threshold=500)
     assert st["n_overflow"] >= 7
     check_db(eng, db, q, min_len=1)                           # everything in pairs
+
+
+# --- many queries in one launch ---------------------------------------------------
+
+def test_batch_equals_single_and_oracle(eng):
+    rng = np.random.Generator(np.random.PCG64(81))
+    db = long_db(82, n=150, lo=20, hi=4000)
+    qs = [G.random_residues(rng, int(m)) for m in (1, 30, 144, 333, 1000, 1500, 17, 2048)]
+    qs.append(G.mutate(rng, db.seq(3)))                      # one strong hit
+    d = eng.load_db(db, group_min_len=1200)
+    got, st = eng.search_batch(d, qs)
+    assert got.shape == (len(qs), db.n)
+    for i, q in enumerate(qs):
+        want = O.search_i32(q, db, eng.matrix, 10, 2)
+        assert np.array_equal(got[i], want), f"query {i} (len {len(q)})"
+        single, _ = eng.search(d, q)
+        assert np.array_equal(single, want)
+    assert st["cells"] == sum(db.total * len(q) for q in qs)
+    forced, st2 = eng.search_batch(d, qs, overflow_threshold=200)
+    assert st2["n_overflow"] > 0
+    assert np.array_equal(forced, got)
+    d.close()
+
+
+def test_batch_empty_and_one(eng):
+    db = small_db(50, 83)
+    d = eng.load_db(db)
+    got, st = eng.search_batch(d, [])
+    assert got.shape == (0, db.n)
+    q = G.queries([64], 84)[0]
+    got, _ = eng.search_batch(d, [q])
+    assert np.array_equal(got[0], O.search_i32(q, db, eng.matrix, 10, 2))
