@@ -43,9 +43,7 @@ typedef struct swh_db swh_db;
 
 /* Timing and counters of the last search on a context. */
 typedef struct swh_stats {
-    double kernel_ms;         /* int16 scan kernels (HIP events on the search stream) */
-    double group_ms;          /*   of which the group-mode scan (long sequences) */
-    double lane_ms;           /*   of which the lane-mode scan */
+    double kernel_ms;         /* int16 scan kernel (HIP events on the search stream) */
     double rescore_ms;        /* int32 rescore kernel */
     double total_ms;          /* whole call, host wall clock */
     uint64_t cells;           /* sum over sequences of len(db seq) * len(query) */

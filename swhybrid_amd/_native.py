@@ -27,8 +27,6 @@ EXPORTS = (
 class SwhStats(C.Structure):
     _fields_ = [
         ("kernel_ms", C.c_double),
-        ("group_ms", C.c_double),
-        ("lane_ms", C.c_double),
         ("rescore_ms", C.c_double),
         ("total_ms", C.c_double),
         ("cells", C.c_uint64),

@@ -120,12 +120,12 @@ int check_scoring(const int32_t* matrix, int gap_open, int gap_extend) {
 struct swh_ctx {
     int device = 0;
     int num_cus = 0;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;     // every launch and copy of the context
     int32_t matrix[1024];
     int gap_open = 10, gap_extend = 2, max_score = 0;
     DevBuf border, counters, prof, scores, ovf, desc;
     PinnedBuf h_prof, h_scores, h_desc, h_ctr;
-    hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t ev[8] = {};
     size_t border_budget = size_t(8) << 30;   // bytes of HBM for strip borders
 };
 
@@ -176,7 +176,7 @@ int swh_ctx_create(int device, const int32_t* matrix, int gap_open, int gap_exte
     }
     c->num_cus = prop.multiProcessorCount;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
-        delete c;
+        swh_ctx_destroy(c);
         return fail(SWH_ERR_HIP, "hipStreamCreate failed");
     }
     for (auto& e : c->ev) (void)hipEventCreate(&e);
@@ -420,32 +420,24 @@ static int run_queries(swh_ctx* c, swh_db* db, const uint8_t* const* qs, const i
         if (stats) *stats = st;
         return SWH_OK;
     }
-    // strip widths shared by the batch: least padded work
+    // strip widths shared by the batch.  Lane scan: least padded work, each
+    // width weighted by its measured cost per padded column on MI355X
+    // (5M-sequence database: W=32 is ~1.5% faster than 24 on long queries,
+    // W=16 ~4% slower).  Group scan: W=16 measured best for every length.
     int Wl = o.strip_width;
     if (!Wl) {
-        uint64_t best = UINT64_MAX;
-        for (int w : {16, 24, 32}) {
-            uint64_t cols = 0;
-            for (int i = 0; i < nq; ++i) cols += (uint64_t)((qlens[i] + w - 1) / w) * w;
+        double best = 1e300;
+        for (int w : {24, 32, 16}) {
+            const double cost = w == 24 ? 1.0 : (w == 32 ? 0.985 : 1.04);
+            double cols = 0;
+            for (int i = 0; i < nq; ++i) cols += (double)((qlens[i] + w - 1) / w) * w * cost;
             if (cols < best) {
                 best = cols;
                 Wl = w;
             }
         }
     }
-    int Wg = o.group_width;
-    if (!Wg) {
-        double best = -1;
-        for (int w : {16, 8}) {
-            double e = 0;
-            int g;
-            for (int i = 0; i < nq; ++i) e += group_eff(qlens[i], w, o.group_lanes, &g) * qlens[i];
-            if (e > best) {
-                best = e;
-                Wg = w;
-            }
-        }
-    }
+    int Wg = o.group_width ? o.group_width : 16;
     std::vector<Plan> plan(nq);
     size_t prof_elems = 0;
     int max_qpad_l = 0, max_pass_g = 0, max_gpw = 1;
@@ -489,7 +481,7 @@ static int run_queries(swh_ctx* c, swh_db* db, const uint8_t* const* qs, const i
     HIP_TRY(c->ovf.reserve((size_t)nq * nseq * sizeof(uint32_t)));
     const size_t n_ctr = 16 + (size_t)nq;
     HIP_TRY(c->counters.reserve(n_ctr * sizeof(uint32_t)));
-    uint32_t* ctr = (uint32_t*)c->counters.p;   // [0] lane, [1] group, [2] rescore; [16+q] overflow count
+    uint32_t* ctr = (uint32_t*)c->counters.p;   // [0] scan, [2] rescore; [16+q] overflow count
     HIP_TRY(launch_reset(ctr, (int)n_ctr, c->stream));
     const int exact_thr = 65535 - c->max_score;
     const int thr = (o.overflow_threshold > 0) ? std::min(o.overflow_threshold, exact_thr)
@@ -537,61 +529,44 @@ static int run_queries(swh_ctx* c, swh_db* db, const uint8_t* const* qs, const i
     const QueryDesc* d_group = d_lane + nq;
     const QueryDesc* d_res = d_lane + 2 * nq;
 
-    // launch shapes and strip-border scratch (the scans run one after the other)
+    // one persistent launch: group items (long pairs) first, then lane items
     const int ps_l = std::min(max_qpad_l, kSegCols) + kProfPad;
-    const size_t lds_l = (size_t)kProfRows * ps_l * sizeof(int16_t);
-    const uint64_t stride_l = lane_border ? (uint64_t)db->max_rows_lane * kWave : 0;
-    const uint64_t per_wg_l = stride_l * sizeof(uint2) * kWavesPerWG;
-    int grid_l = c->num_cus * lane_scan_max_wg_per_cu(Wl, lds_l);
-    if (per_wg_l)
-        grid_l = (int)std::min<uint64_t>(grid_l, std::max<uint64_t>(1, c->border_budget / per_wg_l));
-    grid_l = std::max(1, std::min(grid_l, items_l));
-
     const int ps_g = max_pass_g + kProfPad;
-    const size_t lds_g = (size_t)kProfRows * ps_g * sizeof(int16_t);
+    const size_t lds = (size_t)kProfRows * std::max(ps_l, ps_g) * sizeof(int16_t);
+    const uint64_t stride_l = lane_border ? (uint64_t)db->max_rows_lane * kWave : 0;
     const uint64_t stride_g = group_border ? (uint64_t)max_gpw * db->max_rows_pair : 0;
-    const uint64_t per_wg_g = stride_g * sizeof(uint2) * kWavesPerWG;
-    int grid_g = c->num_cus * group_scan_max_wg_per_cu(Wg, lds_g);
-    if (per_wg_g)
-        grid_g = (int)std::min<uint64_t>(grid_g, std::max<uint64_t>(1, c->border_budget / per_wg_g));
-    grid_g = std::max(1, std::min(grid_g, items_g));
-    HIP_TRY(c->border.reserve(std::max<uint64_t>(
-        {16, per_wg_l * (uint64_t)grid_l, per_wg_g * (uint64_t)grid_g})));
+    const uint64_t stride = std::max(stride_l, stride_g);
+    const uint64_t per_wg = stride * sizeof(uint2) * kWavesPerWG;
+    const int items = items_g + items_l;
+    int grid = c->num_cus * scan_max_wg_per_cu(Wl, Wg, lds);
+    if (per_wg)
+        grid = (int)std::min<uint64_t>(grid, std::max<uint64_t>(1, c->border_budget / per_wg));
+    grid = std::max(1, std::min(grid, items));
+    HIP_TRY(c->border.reserve(std::max<uint64_t>(16, per_wg * (uint64_t)grid)));
 
     ScanArgs a{};
     a.words = (const uint32_t*)db->words.p;
     a.blocks = (const BlockDesc*)db->d_blocks.p;
     a.perm = (const uint32_t*)db->perm.p;
+    a.group_q = d_group;
+    a.lane_q = d_lane;
+    a.work_counter = ctr + 0;
     a.border = (uint2*)c->border.p;
+    a.border_stride = stride;
+    a.border_rows = db->max_rows_pair;
     a.n_queries = nq;
+    a.n_items_group = db->n_pairs > 0 ? items_g : 0;
+    a.n_items_lane = n_lane > 0 ? items_l : 0;
+    a.pair_end = db->n_pairs;
+    a.lane_begin = db->n_pairs;
+    a.lane_end = db->n_blocks;
+    a.ps_lane = ps_l;
+    a.ps_group = ps_g;
     a.gap_oe = gap_oe;
     a.gap_e = c->gap_extend;
-
     HIP_TRY(hipEventRecord(c->ev[0], c->stream));
-    if (db->n_pairs > 0 && items_g > 0) {
-        ScanArgs g = a;
-        g.queries = d_group;
-        g.work_counter = ctr + 1;
-        g.border_stride = stride_g;
-        g.border_rows = db->max_rows_pair;
-        g.n_items = items_g;
-        g.blk_begin = 0;
-        g.blk_end = db->n_pairs;
-        g.ps = ps_g;
-        HIP_TRY(launch_group_scan(Wg, g, grid_g, lds_g, c->stream));
-        st.n_launches += 1;
-    }
-    HIP_TRY(hipEventRecord(c->ev[4], c->stream));
-    if (n_lane > 0 && items_l > 0) {
-        ScanArgs l = a;
-        l.queries = d_lane;
-        l.work_counter = ctr + 0;
-        l.border_stride = stride_l;
-        l.n_items = items_l;
-        l.blk_begin = db->n_pairs;
-        l.blk_end = db->n_blocks;
-        l.ps = ps_l;
-        HIP_TRY(launch_lane_scan(Wl, l, grid_l, lds_l, c->stream));
+    if (a.n_items_group + a.n_items_lane > 0) {
+        HIP_TRY(launch_scan(Wl, Wg, a, grid, lds, c->stream));
         st.n_launches += 1;
     }
     HIP_TRY(hipEventRecord(c->ev[1], c->stream));
@@ -661,10 +636,6 @@ static int run_queries(swh_ctx* c, swh_db* db, const uint8_t* const* qs, const i
     float ms = 0;
     HIP_TRY(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
     st.kernel_ms = ms;
-    HIP_TRY(hipEventElapsedTime(&ms, c->ev[0], c->ev[4]));
-    st.group_ms = ms;
-    HIP_TRY(hipEventElapsedTime(&ms, c->ev[4], c->ev[1]));
-    st.lane_ms = ms;
     if (items_r > 0) {
         HIP_TRY(hipEventElapsedTime(&ms, c->ev[2], c->ev[3]));
         st.rescore_ms = ms;

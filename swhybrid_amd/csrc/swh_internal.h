@@ -46,26 +46,29 @@ struct QueryDesc {
 
 struct ScanArgs {
     const uint32_t* words;     // packed residues
-    const BlockDesc* blocks;   // block table
+    const BlockDesc* blocks;   // block table: pair blocks, then lane blocks
     const uint32_t* perm;      // sorted position -> caller index (kNoSeq for fillers)
-    const QueryDesc* queries;
+    const QueryDesc* group_q;  // per query: group-item range, G, qpad for the group width
+    const QueryDesc* lane_q;   // per query: lane-item range, qpad for the lane width
     uint32_t* work_counter;
-    uint2* border;             // per wave slot (see the kernels)
-    uint64_t border_stride;    // uint2 elements per wave slot
-    uint32_t border_rows;      // group kernel: rows reserved per group
+    uint2* border;             // per wave slot, border_stride uint2
+    uint64_t border_stride;
+    uint32_t border_rows;      // group items: rows reserved per group (longest pair)
     int n_queries;
-    int n_items;               // all queries
-    int blk_begin, blk_end;    // block range of this launch
-    int ps;                    // LDS profile row stride (int16)
+    int n_items_group, n_items_lane;
+    int pair_end;              // pair blocks are [0, pair_end)
+    int lane_begin, lane_end;  // lane blocks
+    int ps_lane, ps_group;     // LDS profile row strides (int16)
     int gap_oe, gap_e;
 };
 
-// Lane kernel: a work item is 4 consecutive lane blocks (one per wave) of one
-// query.  Group kernel: G lanes cooperate on one sequence pair; lane g of a
-// group holds query columns [pass*G*W + g*W, +W) and works on row t-g at step
-// t, the H/F of its last column reaching lane g+1 one step later through a
-// wave-wide DPP shift; a wave holds floor(64/G) groups and a work item is 4
-// waves of pairs.  Rescore: a work item is 256 overflow-list entries.
+// Work items of the search kernel: group items first (G lanes cooperate on
+// one long sequence pair; lane g of a group holds query columns
+// [pass*G*W + g*W, +W) and works on row t-g at step t, its last column's H/F
+// reaching lane g+1 one step later through a wave-wide DPP shift; a wave
+// holds floor(64/G) groups, an item is 4 waves of pairs), then lane items (4
+// consecutive lane blocks of one query, one per wave).  Rescore items are 256
+// overflow-list entries.
 
 struct RescoreArgs {
     const uint32_t* words;
@@ -86,13 +89,10 @@ struct RescoreArgs {
 
 // Launchers (swh_kernels.hip).
 hipError_t launch_reset(uint32_t* p, int n, hipStream_t s);
-hipError_t launch_lane_scan(int width, const ScanArgs& a, int grid, size_t lds_bytes,
-                            hipStream_t s);
-hipError_t launch_group_scan(int width, const ScanArgs& a, int grid, size_t lds_bytes,
-                             hipStream_t s);
-int group_scan_max_wg_per_cu(int width, size_t lds_bytes);
+hipError_t launch_scan(int lane_width, int group_width, const ScanArgs& a, int grid,
+                       size_t lds_bytes, hipStream_t s);
+int scan_max_wg_per_cu(int lane_width, int group_width, size_t lds_bytes);
 hipError_t launch_rescore(const RescoreArgs& a, int grid, size_t lds_bytes, hipStream_t s);
-int lane_scan_max_wg_per_cu(int width, size_t lds_bytes);
 int rescore_max_wg_per_cu(size_t lds_bytes);
 
 }  // namespace swh

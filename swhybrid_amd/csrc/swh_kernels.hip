@@ -56,13 +56,10 @@ __device__ __forceinline__ void stage_profile(short* lds, const int16_t* __restr
 }
 
 // One row of a strip: W cells, packed pairs.  P[c] = (s(q_c, a), s(q_c, b)).
-// h/e/hq hold H(i-1,c), E(i-1,c), H(i-1,c)-(open+ext) on entry and row i on
-// exit; M[c] is the running maximum of column c (independent chains, reduced
-// once per strip, so no serial max chain forms across the row).
+// h/e/hq hold H(i-1,c), E(i-1,c), H(i-1,c)-(open+ext) on entry and row i on exit.
 template <int W, typename T>
-__device__ __forceinline__ void row_cells(T (&h)[W], T (&e)[W], T (&hq)[W], T (&M)[W],
-                                          const T (&P)[W], T diag0, T hl, T fl, T Q, T R,
-                                          T& fout) {
+__device__ __forceinline__ void row_cells(T (&h)[W], T (&e)[W], T (&hq)[W], const T (&P)[W],
+                                          T diag0, T hl, T fl, T Q, T R, T& fout) {
     T X = adds(diag0, P[0]);      // H(i-1, c0-1) + s
     T f = fl;                      // F(i, c0-1)
     T hqp = subs(hl, Q);           // H(i, c0-1) - (open+ext)
@@ -73,7 +70,6 @@ __device__ __forceinline__ void row_cells(T (&h)[W], T (&e)[W], T (&hq)[W], T (&
         T Xn = X;
         if (c + 1 < W) Xn = adds(h[c], P[c + 1]);  // diagonal term of column c+1
         T hn = vmax(vmax(X, e[c]), f);             // H(i, c); zero floor by saturation
-        M[c] = vmax(M[c], hn);
         hq[c] = subs(hn, Q);
         h[c] = hn;
         hqp = hq[c];
@@ -82,23 +78,46 @@ __device__ __forceinline__ void row_cells(T (&h)[W], T (&e)[W], T (&hq)[W], T (&
     fout = f;
 }
 
+// Maximum of a row as a balanced tree (W-1 independent-ish max ops, no state
+// carried between rows), folded into the running score once per row.
 template <int W, typename T>
-__device__ __forceinline__ T reduce_max(const T (&M)[W]) {
-    T m = M[0];
+__device__ __forceinline__ T row_max(const T (&h)[W]) {
+    T m[W];
 #pragma unroll
-    for (int c = 1; c < W; ++c) m = vmax(m, M[c]);
-    return m;
+    for (int c = 0; c < W; ++c) m[c] = h[c];
+#pragma unroll
+    for (int step = 1; step < W; step <<= 1)
+#pragma unroll
+        for (int c = 0; c + step < W; c += 2 * step) m[c] = vmax(m[c], m[c + step]);
+    return m[0];
 }
 
+// Raw profile rows of the lane's two residues (W int16 each), and their
+// interleave into (s(q_c, a), s(q_c, b)) pairs with one v_perm_b32 per column.
 template <int W>
-__device__ __forceinline__ void pair_profile(s2 (&P)[W], const short* lds, int ps, int a, int b,
-                                             int jj0) {
+struct RawProfile {
+    uint4 va[W / 8];
+    uint4 vb[W / 8];
+};
+
+template <int W>
+__device__ __forceinline__ void load_profile(RawProfile<W>& rp, const short* lds, int ps, int a,
+                                             int b, int jj0) {
     const uint4* pa = reinterpret_cast<const uint4*>(lds + __umul24(a, ps) + jj0);
     const uint4* pb = reinterpret_cast<const uint4*>(lds + __umul24(b, ps) + jj0);
 #pragma unroll
     for (int k = 0; k < W / 8; ++k) {
-        const uint4 va = pa[k];
-        const uint4 vb = pb[k];
+        rp.va[k] = pa[k];
+        rp.vb[k] = pb[k];
+    }
+}
+
+template <int W>
+__device__ __forceinline__ void interleave_profile(s2 (&P)[W], const RawProfile<W>& rp) {
+#pragma unroll
+    for (int k = 0; k < W / 8; ++k) {
+        const uint4 va = rp.va[k];
+        const uint4 vb = rp.vb[k];
         P[8 * k + 0] = as_s2(__builtin_amdgcn_perm(vb.x, va.x, 0x05040100u));
         P[8 * k + 1] = as_s2(__builtin_amdgcn_perm(vb.x, va.x, 0x07060302u));
         P[8 * k + 2] = as_s2(__builtin_amdgcn_perm(vb.y, va.y, 0x05040100u));
@@ -110,56 +129,84 @@ __device__ __forceinline__ void pair_profile(s2 (&P)[W], const short* lds, int p
     }
 }
 
-// One strip of W columns over every row of a lane-mode block.
 template <int W>
+__device__ __forceinline__ void pair_profile(s2 (&P)[W], const short* lds, int ps, int a, int b,
+                                             int jj0) {
+    RawProfile<W> rp;
+    load_profile<W>(rp, lds, ps, a, b, jj0);
+    interleave_profile<W>(P, rp);
+}
+
+// One strip of W columns over every row of a lane-mode block.  IN / OUT: the
+// strip reads / writes the border array (compile-time, so that a 4-row tile is
+// a single basic block and the compiler keeps each row's work in place).
+template <int W, bool IN, bool OUT>
 __device__ __forceinline__ void lane_strip(const uint2* __restrict__ wds, int quads,
                                            uint2* __restrict__ bord, const short* lds, int ps,
-                                           int jj0, bool has_in, bool has_out, s2 Q, s2 R,
-                                           s2& S, int lane) {
+                                           int jj0, s2 Q, s2 R, s2& S, int lane) {
     const s2 Z = as_s2(kZero16x2);
-    s2 h[W], e[W], hq[W], M[W];
+    s2 h[W], e[W], hq[W];
 #pragma unroll
     for (int c = 0; c < W; ++c) {
         h[c] = Z;
         e[c] = Z;
         hq[c] = Z;
-        M[c] = Z;
     }
     s2 diag0 = Z;
     const uint2 zz = make_uint2(kZero16x2, kZero16x2);
     uint2 wn = wds[lane];
     uint2 bn[4];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) bn[r] = has_in ? bord[r * kWave + lane] : zz;
+    for (int r = 0; r < 4; ++r) bn[r] = IN ? bord[r * kWave + lane] : zz;
+    RawProfile<W> rp;
+    load_profile<W>(rp, lds, ps, wn.x & 0xff, wn.y & 0xff, jj0);
     for (int k = 0; k < quads; ++k) {
         const uint2 w = wn;
         uint2 bc[4];
 #pragma unroll
         for (int r = 0; r < 4; ++r) bc[r] = bn[r];
-        if (k + 1 < quads) {
-            wn = wds[(size_t)(k + 1) * kWave + lane];
-            if (has_in) {
-                const uint2* bp = bord + (size_t)(k + 1) * 4 * kWave + lane;
+        // prefetch the next tile (the last tile reloads itself: no branch)
+        const int kn = min(k + 1, quads - 1);
+        wn = wds[(size_t)kn * kWave + lane];
+        if (IN) {
+            const uint2* bp = bord + (size_t)kn * 4 * kWave + lane;
 #pragma unroll
-                for (int r = 0; r < 4; ++r) bn[r] = bp[r * kWave];
-            }
+            for (int r = 0; r < 4; ++r) bn[r] = bp[r * kWave];
         }
         uint2* bo = bord + (size_t)k * 4 * kWave + lane;
+        // the profile of row r+1 is loaded while row r computes; row barriers
+        // keep the scheduler from hoisting every load of the tile at once
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            const int a = (w.x >> (8 * r)) & 0xff;
-            const int b = (w.y >> (8 * r)) & 0xff;
+            const RawProfile<W> cur = rp;
+            const uint2 nw = r < 3 ? w : wn;
+            const int sh = 8 * ((r + 1) & 3);
+            load_profile<W>(rp, lds, ps, (nw.x >> sh) & 0xff, (nw.y >> sh) & 0xff, jj0);
             s2 P[W];
-            pair_profile<W>(P, lds, ps, a, b, jj0);
+            interleave_profile<W>(P, cur);
             const s2 hl = as_s2(bc[r].x);
             const s2 fl = as_s2(bc[r].y);
             s2 fout;
-            row_cells<W, s2>(h, e, hq, M, P, diag0, hl, fl, Q, R, fout);
-            if (has_out) bo[r * kWave] = make_uint2(bits(h[W - 1]), bits(fout));
+            row_cells<W, s2>(h, e, hq, P, diag0, hl, fl, Q, R, fout);
+            S = vmax(S, row_max<W, s2>(h));
+            if (OUT) bo[r * kWave] = make_uint2(bits(h[W - 1]), bits(fout));
             diag0 = hl;
+            __builtin_amdgcn_sched_barrier(0);
         }
     }
-    S = vmax(S, reduce_max<W, s2>(M));
+}
+
+template <int W>
+__device__ __forceinline__ void lane_strip_any(const uint2* wds, int quads, uint2* bord,
+                                               const short* lds, int ps, int jj0, bool has_in,
+                                               bool has_out, s2 Q, s2 R, s2& S, int lane) {
+    if (has_in) {
+        if (has_out) lane_strip<W, true, true>(wds, quads, bord, lds, ps, jj0, Q, R, S, lane);
+        else lane_strip<W, true, false>(wds, quads, bord, lds, ps, jj0, Q, R, S, lane);
+    } else {
+        if (has_out) lane_strip<W, false, true>(wds, quads, bord, lds, ps, jj0, Q, R, S, lane);
+        else lane_strip<W, false, false>(wds, quads, bord, lds, ps, jj0, Q, R, S, lane);
+    }
 }
 
 __device__ __forceinline__ void emit(const QueryDesc& Qd, const uint32_t* perm, uint32_t s,
@@ -199,47 +246,42 @@ __device__ __forceinline__ bool next_item(uint32_t* counter, const QueryDesc* qs
     return item < n_items;
 }
 
+// One lane-mode work item: 4 consecutive lane blocks of query q, one per wave.
 template <int W>
-__global__ __launch_bounds__(256) void k_lane_scan(ScanArgs A) {
-    extern __shared__ __align__(16) short lds[];
-    __shared__ int s_buf[2];
+__device__ __forceinline__ void lane_item(const ScanArgs& A, const QueryDesc& Qd, int q,
+                                          int local, short* lds, uint2* bord, int& loaded_q,
+                                          int& loaded_seg) {
     const int wave = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
-    uint2* bord = A.border + (size_t)(blockIdx.x * kWavesPerWG + wave) * A.border_stride;
     const s2 Q = {(short)A.gap_oe, (short)A.gap_oe};
     const s2 R = {(short)A.gap_e, (short)A.gap_e};
-    int loaded_q = -1, loaded_seg = -1;
-    int item, q;
-    while (next_item(A.work_counter, A.queries, A.n_queries, A.n_items, s_buf, item, q)) {
-        const QueryDesc Qd = A.queries[q];
-        const int blk = A.blk_begin + (item - Qd.item_begin) * kWavesPerWG + wave;
-        const bool active = blk < A.blk_end;
-        BlockDesc bd = {0, 0, 0};
-        if (active) bd = A.blocks[blk];
-        const uint2* wds = reinterpret_cast<const uint2*>(A.words + bd.word_off);
-        const int quads = (int)(bd.rows >> 2);
-        const int nseg = (Qd.qpad + kSegCols - 1) / kSegCols;
-        s2 S = as_s2(kZero16x2);
-        for (int seg = 0; seg < nseg; ++seg) {
-            const int seg0 = seg * kSegCols;
-            const int segw = min(kSegCols, Qd.qpad - seg0);
-            if (q != loaded_q || seg != loaded_seg) {
-                __syncthreads();
-                stage_profile(lds, Qd.prof, Qd.pstride, A.ps, seg0, segw);
-                __syncthreads();
-                loaded_q = q;
-                loaded_seg = seg;
-            }
-            if (active) {
-                for (int c0 = seg0; c0 < seg0 + segw; c0 += W)
-                    lane_strip<W>(wds, quads, bord, lds, A.ps, c0 - seg0, c0 > 0,
-                                  c0 + W < Qd.qpad, Q, R, S, lane);
-            }
+    const int blk = A.lane_begin + local * kWavesPerWG + wave;
+    const bool active = blk < A.lane_end;
+    BlockDesc bd = {0, 0, 0};
+    if (active) bd = A.blocks[blk];
+    const uint2* wds = reinterpret_cast<const uint2*>(A.words + bd.word_off);
+    const int quads = (int)(bd.rows >> 2);
+    const int nseg = (Qd.qpad + kSegCols - 1) / kSegCols;
+    s2 S = as_s2(kZero16x2);
+    for (int seg = 0; seg < nseg; ++seg) {
+        const int seg0 = seg * kSegCols;
+        const int segw = min(kSegCols, Qd.qpad - seg0);
+        if (q != loaded_q || seg != loaded_seg) {
+            __syncthreads();
+            stage_profile(lds, Qd.prof, Qd.pstride, A.ps_lane, seg0, segw);
+            __syncthreads();
+            loaded_q = q;
+            loaded_seg = seg;
         }
         if (active) {
-            emit(Qd, A.perm, bd.first + lane, (int)S.x + 32768);
-            emit(Qd, A.perm, bd.first + kWave + lane, (int)S.y + 32768);
+            for (int c0 = seg0; c0 < seg0 + segw; c0 += W)
+                lane_strip_any<W>(wds, quads, bord, lds, A.ps_lane, c0 - seg0, c0 > 0,
+                                  c0 + W < Qd.qpad, Q, R, S, lane);
         }
+    }
+    if (active) {
+        emit(Qd, A.perm, bd.first + lane, (int)S.x + 32768);
+        emit(Qd, A.perm, bd.first + kWave + lane, (int)S.y + 32768);
     }
 }
 
@@ -256,114 +298,163 @@ __device__ __forceinline__ uint32_t wave_shr1(uint32_t v) {
 }
 
 template <int W>
-__global__ __launch_bounds__(256) void k_group_scan(ScanArgs A) {
-    extern __shared__ __align__(16) short lds[];
-    __shared__ int s_buf[2];
-    __shared__ uint32_t s_red[kWavesPerWG][kWave];
+__device__ __forceinline__ void group_item(const ScanArgs& A, const QueryDesc& Qd, int local,
+                                           short* lds, uint2* slot, uint32_t (*s_red)[kWave]) {
     const int wave = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
-    uint2* slot = A.border + (size_t)(blockIdx.x * kWavesPerWG + wave) * A.border_stride;
     const s2 Q = {(short)A.gap_oe, (short)A.gap_oe};
     const s2 R = {(short)A.gap_e, (short)A.gap_e};
     const s2 Z = as_s2(kZero16x2);
-    int item, q;
-    while (next_item(A.work_counter, A.queries, A.n_queries, A.n_items, s_buf, item, q)) {
-        const QueryDesc Qd = A.queries[q];
-        const int G = Qd.group;
-        const int gpw = kWave / G;
-        const int gi = lane / G;
-        const int gl = lane - gi * G;
-        const bool in_group = gi < gpw;
-        uint2* bord = slot + (size_t)(in_group ? gi : 0) * A.border_rows;
-        const int p = A.blk_begin + ((item - Qd.item_begin) * kWavesPerWG + wave) * gpw + gi;
-        const bool has_pair = in_group && p < A.blk_end;
-        BlockDesc bd = {0, 0, 0};
-        if (has_pair) bd = A.blocks[p];
-        uint32_t rmax = bd.rows;
-        for (int off = 32; off > 0; off >>= 1)
-            rmax = max(rmax, (uint32_t)__shfl_xor((int)rmax, off));
-        rmax = __builtin_amdgcn_readfirstlane(rmax);   // equal in every lane: keep loops scalar
-        const uint2* pw = reinterpret_cast<const uint2*>(A.words + bd.word_off);
-        const int pass_cols = G * W;
-        const int npass = (Qd.qpad + pass_cols - 1) / pass_cols;
-        s2 S = Z;
-        for (int pass = 0; pass < npass; ++pass) {
-            const int col_base = pass * pass_cols;
-            const int pcols = min(pass_cols, Qd.qpad - col_base);
-            const int nl = pcols / W;
-            __syncthreads();
-            stage_profile(lds, Qd.prof, Qd.pstride, A.ps, col_base, pcols);
-            __syncthreads();
-            const bool has_in = pass > 0;
-            const bool has_out = pass + 1 < npass;
-            const bool on = has_pair && gl < nl;
-            const int jj0 = gl * W;
-            s2 h[W], e[W], hq[W], M[W];
+    const int G = Qd.group;
+    const int gpw = kWave / G;
+    const int gi = lane / G;
+    const int gl = lane - gi * G;
+    const bool in_group = gi < gpw;
+    uint2* bord = slot + (size_t)(in_group ? gi : 0) * A.border_rows;
+    const int p = (local * kWavesPerWG + wave) * gpw + gi;
+    const bool has_pair = in_group && p < A.pair_end;
+    BlockDesc bd = {0, 0, 0};
+    if (has_pair) bd = A.blocks[p];
+    uint32_t rmax = bd.rows;
+    for (int off = 32; off > 0; off >>= 1) rmax = max(rmax, (uint32_t)__shfl_xor((int)rmax, off));
+    rmax = __builtin_amdgcn_readfirstlane(rmax);   // equal in every lane: keep loops scalar
+    const uint2* pw = reinterpret_cast<const uint2*>(A.words + bd.word_off);
+    const int pass_cols = G * W;
+    const int npass = (Qd.qpad + pass_cols - 1) / pass_cols;
+    s2 S = Z;
+    for (int pass = 0; pass < npass; ++pass) {
+        const int col_base = pass * pass_cols;
+        const int pcols = min(pass_cols, Qd.qpad - col_base);
+        const int nl = pcols / W;
+        __syncthreads();
+        stage_profile(lds, Qd.prof, Qd.pstride, A.ps_group, col_base, pcols);
+        __syncthreads();
+        const bool has_in = pass > 0;
+        const bool has_out = pass + 1 < npass;
+        const bool on = has_pair && gl < nl;
+        const int jj0 = gl * W;
+        s2 h[W], e[W], hq[W];
 #pragma unroll
-            for (int c = 0; c < W; ++c) {
-                h[c] = Z;
-                e[c] = Z;
-                hq[c] = Z;
-                M[c] = Z;
-            }
-            s2 diag0 = Z;
-            uint32_t ch = kZero16x2, cf = kZero16x2;   // this lane's last (H, F)
-            const int steps = rmax > 0 ? (int)rmax + nl - 1 : 0;
-            // residue words and border entries are loaded 4 steps ahead
-            const bool reads_border = has_in && gl == 0 && has_pair;
-            uint2 wn[4], bn[4];
+        for (int c = 0; c < W; ++c) {
+            h[c] = Z;
+            e[c] = Z;
+            hq[c] = Z;
+        }
+        s2 diag0 = Z;
+        uint32_t ch = kZero16x2, cf = kZero16x2;   // this lane's last (H, F)
+        const int steps = rmax > 0 ? (int)rmax + nl - 1 : 0;
+        // residue words and border entries are loaded 4 steps ahead
+        const bool reads_border = has_in && gl == 0 && has_pair;
+        uint2 wn[4], bn[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int row = u - gl;
+            wn[u] = make_uint2(0u, 0u);
+            bn[u] = make_uint2(kZero16x2, kZero16x2);
+            if (on && row >= 0 && row < (int)bd.rows) wn[u] = pw[row >> 2];
+            if (reads_border && u < (int)bd.rows) bn[u] = bord[u];
+        }
+        for (int t0 = 0; t0 < steps; t0 += 4) {
+            uint2 wc[4], bc[4];
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
-                const int row = u - gl;
-                wn[u] = make_uint2(0u, 0u);
-                bn[u] = make_uint2(kZero16x2, kZero16x2);
+                wc[u] = wn[u];
+                bc[u] = bn[u];
+                const int tn = t0 + 4 + u;
+                const int row = tn - gl;
                 if (on && row >= 0 && row < (int)bd.rows) wn[u] = pw[row >> 2];
-                if (reads_border && u < (int)bd.rows) bn[u] = bord[u];
+                if (reads_border && tn < (int)bd.rows) bn[u] = bord[tn];
             }
-            for (int t0 = 0; t0 < steps; t0 += 4) {
-                uint2 wc[4], bc[4];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    wc[u] = wn[u];
-                    bc[u] = bn[u];
-                    const int tn = t0 + 4 + u;
-                    const int row = tn - gl;
-                    if (on && row >= 0 && row < (int)bd.rows) wn[u] = pw[row >> 2];
-                    if (reads_border && tn < (int)bd.rows) bn[u] = bord[tn];
-                }
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const int row = t0 + u - gl;
-                    // every lane takes part in the shift (no lane may be masked off here)
-                    const uint32_t hx = wave_shr1(ch);
-                    const uint32_t fx = wave_shr1(cf);
-                    const uint32_t hl_u = gl == 0 ? bc[u].x : hx;
-                    const uint32_t fl_u = gl == 0 ? bc[u].y : fx;
-                    if (on && row >= 0 && row < (int)bd.rows) {
-                        const int sh = 8 * (row & 3);
-                        const int a = (wc[u].x >> sh) & 0xff;
-                        const int b = (wc[u].y >> sh) & 0xff;
-                        s2 P[W];
-                        pair_profile<W>(P, lds, A.ps, a, b, jj0);
-                        s2 fout;
-                        row_cells<W, s2>(h, e, hq, M, P, diag0, as_s2(hl_u), as_s2(fl_u), Q, R,
-                                         fout);
-                        ch = bits(h[W - 1]);
-                        cf = bits(fout);
-                        diag0 = as_s2(hl_u);
-                        if (has_out && gl == nl - 1) bord[row] = make_uint2(ch, cf);
-                    }
+            for (int u = 0; u < 4; ++u) {
+                const int row = t0 + u - gl;
+                // every lane takes part in the shift (no lane may be masked off here)
+                const uint32_t hx = wave_shr1(ch);
+                const uint32_t fx = wave_shr1(cf);
+                const uint32_t hl_u = gl == 0 ? bc[u].x : hx;
+                const uint32_t fl_u = gl == 0 ? bc[u].y : fx;
+                if (on && row >= 0 && row < (int)bd.rows) {
+                    const int sh = 8 * (row & 3);
+                    const int a = (wc[u].x >> sh) & 0xff;
+                    const int b = (wc[u].y >> sh) & 0xff;
+                    s2 P[W];
+                    pair_profile<W>(P, lds, A.ps_group, a, b, jj0);
+                    s2 fout;
+                    row_cells<W, s2>(h, e, hq, P, diag0, as_s2(hl_u), as_s2(fl_u), Q, R, fout);
+                    S = vmax(S, row_max<W, s2>(h));
+                    ch = bits(h[W - 1]);
+                    cf = bits(fout);
+                    diag0 = as_s2(hl_u);
+                    if (has_out && gl == nl - 1) bord[row] = make_uint2(ch, cf);
                 }
             }
-            if (on) S = vmax(S, reduce_max<W, s2>(M));
         }
-        s_red[wave][lane] = bits(S);
-        __syncthreads();
-        if (has_pair && gl == 0) {
-            s2 m = S;
-            for (int k = 1; k < G; ++k) m = vmax(m, as_s2(s_red[wave][lane + k]));
-            emit(Qd, A.perm, bd.first, (int)m.x + 32768);
-            emit(Qd, A.perm, bd.first + 1, (int)m.y + 32768);
+    }
+    s_red[wave][lane] = bits(S);
+    __syncthreads();
+    if (has_pair && gl == 0) {
+        s2 m = S;
+        for (int k = 1; k < G; ++k) m = vmax(m, as_s2(s_red[wave][lane + k]));
+        emit(Qd, A.perm, bd.first, (int)m.x + 32768);
+        emit(Qd, A.perm, bd.first + 1, (int)m.y + 32768);
+    }
+}
+
+// The search kernel: one persistent launch, one device work queue.  Items
+// [0, n_items_group) are group items (long pairs, longest first), then
+// [n_items_group, +n_items_lane) lane items; each range is query-major and
+// the query of an item is found in the group or lane query table.  Long work
+// therefore starts first and short blocks fill in around it.
+__device__ __forceinline__ bool next_scan_item(const ScanArgs& A, int* s_buf, int& local,
+                                               int& q, bool& is_group) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int it = (int)atomicAdd(A.work_counter, 1u);
+        int kind = 0, rel = it;
+        const QueryDesc* qs = A.group_q;
+        if (it >= A.n_items_group) {
+            kind = 1;
+            rel = it - A.n_items_group;
+            qs = A.lane_q;
+        }
+        int lo = 0, hi = A.n_queries - 1;
+        const bool live = it < A.n_items_group + A.n_items_lane;
+        if (live) {
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (qs[mid].item_begin <= rel) lo = mid;
+                else hi = mid - 1;
+            }
+        }
+        s_buf[0] = live ? rel - qs[lo].item_begin : -1;
+        s_buf[1] = lo;
+        s_buf[2] = kind;
+    }
+    __syncthreads();
+    local = s_buf[0];
+    q = s_buf[1];
+    is_group = s_buf[2] == 0;
+    return local >= 0;
+}
+
+template <int WL, int WG>
+__global__ __launch_bounds__(256) void k_scan(ScanArgs A) {
+    extern __shared__ __align__(16) short lds[];
+    __shared__ int s_buf[3];
+    __shared__ uint32_t s_red[kWavesPerWG][kWave];
+    const int wave = threadIdx.x >> 6;
+    uint2* slot = A.border + (size_t)(blockIdx.x * kWavesPerWG + wave) * A.border_stride;
+    int loaded_q = -1, loaded_seg = -1;
+    int local, q;
+    bool is_group;
+    while (next_scan_item(A, s_buf, local, q, is_group)) {
+        if (is_group) {
+            const QueryDesc Qd = A.group_q[q];
+            group_item<WG>(A, Qd, local, lds, slot, s_red);
+            loaded_q = -1;   // the group item restaged the profile
+        } else {
+            const QueryDesc Qd = A.lane_q[q];
+            lane_item<WL>(A, Qd, q, local, lds, slot, loaded_q, loaded_seg);
         }
     }
 }
@@ -430,13 +521,12 @@ __global__ __launch_bounds__(256) void k_rescore(RescoreArgs A) {
             for (int c0 = seg0; c0 < seg0 + segw; c0 += W) {
                 const bool has_in = c0 > 0;
                 const bool has_out = c0 + W < Qd.qpad;
-                int h[W], e[W], hq[W], M[W];
+                int h[W], e[W], hq[W];
 #pragma unroll
                 for (int c = 0; c < W; ++c) {
                     h[c] = kZero32;
                     e[c] = kZero32;
                     hq[c] = kZero32;
-                    M[c] = kZero32;
                 }
                 int diag0 = kZero32;
                 for (int k = 0; k < quads; ++k) {
@@ -452,15 +542,15 @@ __global__ __launch_bounds__(256) void k_rescore(RescoreArgs A) {
                         uint2 bin = make_uint2((uint32_t)kZero32, (uint32_t)kZero32);
                         if (has_in) bin = bord[(size_t)row * kWave + lane];
                         int fout;
-                        row_cells<W, int>(h, e, hq, M, P, diag0, (int)bin.x, (int)bin.y,
+                        row_cells<W, int>(h, e, hq, P, diag0, (int)bin.x, (int)bin.y,
                                           A.gap_oe, A.gap_e, fout);
+                        S = vmax(S, row_max<W, int>(h));
                         if (has_out)
                             bord[(size_t)row * kWave + lane] =
                                 make_uint2((uint32_t)h[W - 1], (uint32_t)fout);
                         diag0 = (int)bin.x;
                     }
                 }
-                S = vmax(S, reduce_max<W, int>(M));
             }
         }
         if (valid) {
@@ -481,33 +571,20 @@ hipError_t launch_reset(uint32_t* p, int n, hipStream_t s) {
     return hipGetLastError();
 }
 
-template <int W>
-static hipError_t launch_lane_w(const ScanArgs& a, int grid, size_t lds, hipStream_t s) {
-    hipLaunchKernelGGL(k_lane_scan<W>, dim3(grid), dim3(256), lds, s, a);
+template <int WL, int WG>
+static hipError_t launch_scan_w(const ScanArgs& a, int grid, size_t lds, hipStream_t s) {
+    hipLaunchKernelGGL((k_scan<WL, WG>), dim3(grid), dim3(256), lds, s, a);
     return hipGetLastError();
 }
 
-hipError_t launch_lane_scan(int width, const ScanArgs& a, int grid, size_t lds, hipStream_t s) {
-    switch (width) {
-        case 16: return launch_lane_w<16>(a, grid, lds, s);
-        case 24: return launch_lane_w<24>(a, grid, lds, s);
-        case 32: return launch_lane_w<32>(a, grid, lds, s);
-        default: return hipErrorInvalidValue;
-    }
-}
+#define SWH_SCAN_CASES(X) X(16, 8) X(16, 16) X(24, 8) X(24, 16) X(32, 8) X(32, 16)
 
-template <int W>
-static hipError_t launch_group_w(const ScanArgs& a, int grid, size_t lds, hipStream_t s) {
-    hipLaunchKernelGGL(k_group_scan<W>, dim3(grid), dim3(256), lds, s, a);
-    return hipGetLastError();
-}
-
-hipError_t launch_group_scan(int width, const ScanArgs& a, int grid, size_t lds, hipStream_t s) {
-    switch (width) {
-        case 8: return launch_group_w<8>(a, grid, lds, s);
-        case 16: return launch_group_w<16>(a, grid, lds, s);
-        default: return hipErrorInvalidValue;
-    }
+hipError_t launch_scan(int wl, int wg, const ScanArgs& a, int grid, size_t lds, hipStream_t s) {
+#define X(L, G) \
+    if (wl == L && wg == G) return launch_scan_w<L, G>(a, grid, lds, s);
+    SWH_SCAN_CASES(X)
+#undef X
+    return hipErrorInvalidValue;
 }
 
 hipError_t launch_rescore(const RescoreArgs& a, int grid, size_t lds, hipStream_t s) {
@@ -522,21 +599,12 @@ static int occ(K kern, size_t lds) {
     return n > 0 ? n : 1;
 }
 
-int lane_scan_max_wg_per_cu(int width, size_t lds) {
-    switch (width) {
-        case 16: return occ(k_lane_scan<16>, lds);
-        case 24: return occ(k_lane_scan<24>, lds);
-        case 32: return occ(k_lane_scan<32>, lds);
-        default: return 1;
-    }
-}
-
-int group_scan_max_wg_per_cu(int width, size_t lds) {
-    switch (width) {
-        case 8: return occ(k_group_scan<8>, lds);
-        case 16: return occ(k_group_scan<16>, lds);
-        default: return 1;
-    }
+int scan_max_wg_per_cu(int wl, int wg, size_t lds) {
+#define X(L, G) \
+    if (wl == L && wg == G) return occ(k_scan<L, G>, lds);
+    SWH_SCAN_CASES(X)
+#undef X
+    return 1;
 }
 
 int rescore_max_wg_per_cu(size_t lds) { return occ(k_rescore, lds); }
