@@ -68,6 +68,76 @@ def cpu_baseline(db, queries, matrix, go, ge, stride=16):
     }
 
 
+def run_batched(eng, d, plain_gcups):
+    """64 queries, lengths log-uniform in [50, 5000], one call: one persistent
+    launch whose device work queue holds every (query, block group) item."""
+    from swhybrid_amd import datagen as G
+    qs = G.c3_queries()
+    eng.search_batch(d, qs[:2])                      # warm the launch path
+    t0 = time.perf_counter()
+    scores, st = eng.search_batch(d, qs)
+    wall = time.perf_counter() - t0
+    del scores
+    log(f"batched: {len(qs)} queries, {st['cells']:.3e} cells in {wall:.2f} s")
+    g = st["cells"] / wall / 1e9
+    return {"workload": "64 queries, lengths log-uniform [50, 5000], one call, all scores "
+                        "returned", "queries": len(qs), "sum_qlen": int(sum(len(q) for q in qs)),
+            "gcups": round(g, 1), "kernel_gcups": round(st["cells"] / (st["kernel_ms"] / 1e3) / 1e9, 1),
+            "wall_s": round(wall, 3), "vs_plain": round(g / plain_gcups, 3),
+            "n_overflow": st["n_overflow"]}
+
+
+def run_serving(eng, d, qps_levels, n_requests):
+    """Poisson arrivals at fixed QPS, lengths logN(mean 400, sigma 0.7) clipped
+    to [30, 5000], top-20 per request, the database resident; p50/p99 latency
+    (submit to top-k delivered) and sustained GCUPS per level."""
+    import threading
+    from swhybrid_amd import datagen as G
+    from swhybrid_amd.engine import Server
+    res = {}
+    for qps in qps_levels:
+        arrivals, qs = G.c4_requests(n_requests, qps)
+        srv = Server(eng, d, slices=4)
+        ids = [None] * len(qs)
+        t0 = time.perf_counter()
+
+        def feed():
+            for i, (a, q) in enumerate(zip(arrivals, qs)):
+                delay = t0 + a - time.perf_counter()
+                if delay > 0:
+                    time.sleep(delay)
+                ids[i] = srv.submit(q, k=TOP_K)
+
+        th = threading.Thread(target=feed)
+        th.start()
+        lat, done_ms, first_submit = [], [], None
+        cells = 0
+        for i in range(len(qs)):
+            while ids[i] is None:
+                time.sleep(0.0005)
+            r = srv.wait(ids[i])
+            info = r["info"]
+            lat.append(info["latency_ms"])
+            done_ms.append(info["done_ms"])
+            first_submit = info["submit_ms"] if first_submit is None else min(first_submit, info["submit_ms"])
+            cells += info["cells"]
+        th.join()
+        st = srv.stats()
+        srv.close()
+        span = (max(done_ms) - first_submit) / 1e3
+        lat = np.array(lat)
+        res[str(int(qps)) if float(qps).is_integer() else str(qps)] = {
+            "requests": len(qs), "p50_ms": round(float(np.percentile(lat, 50)), 1),
+            "p99_ms": round(float(np.percentile(lat, 99)), 1),
+            "mean_ms": round(float(lat.mean()), 1),
+            "gcups": round(cells / span / 1e9, 1), "offered_tcups": round(
+                qps * np.mean([len(q) for q in qs]) * d.residues / 1e12, 2),
+            "peak_inflight": st["peak_inflight"], "rounds": st["rounds"]}
+        log(f"serving {qps} QPS: p50 {res[list(res)[-1]]['p50_ms']} ms, "
+            f"p99 {res[list(res)[-1]]['p99_ms']} ms")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -78,6 +148,10 @@ def main():
     ap.add_argument("--group-min-len", type=int, default=0)
     ap.add_argument("--query-lengths", type=str, default="",
                     help="comma-separated subset of the workload's query lengths (profiling runs)")
+    ap.add_argument("--no-batched", action="store_true", help="skip the 64-query batched run")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving run")
+    ap.add_argument("--serve-qps", type=str, default="5,20,50")
+    ap.add_argument("--serve-requests", type=int, default=60)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -166,6 +240,12 @@ def main():
         "per_query": pq_out,
         "top1": {str(k): int(v[0][0]) for k, v in tops.items()},
     }
+    if not args.no_batched:
+        out["batched"] = run_batched(eng, d, gcups)
+    if not args.no_serve:
+        out["served"] = run_serving(eng, d, [float(x) for x in args.serve_qps.split(",")],
+                                    args.serve_requests)
+    d.close()
     eng.close()
     if not args.no_cpu_baseline:
         log("cpu baseline ...")

@@ -117,6 +117,56 @@ int swh_search_batch(swh_ctx* ctx, swh_db* db, const uint8_t* const* queries,
 int swh_topk(const int32_t* scores, uint64_t n, int k, int32_t* out_score,
              int64_t* out_index);
 
+/* ---- Serving: continuous batching on a resident database ------------------
+ * A server owns a loop thread.  Requests are admitted, in arrival order, when
+ * the device memory their query profile, score vector and overflow list need
+ * fits the budget; each round then runs ONE persistent scan launch over the
+ * next database slice of every admitted request, so a request that arrives
+ * while others are on the GPU joins them at the next slice boundary and their
+ * work overlaps on the device.  Scores are exact and independent of which
+ * requests share a round.  No reference counterpart (the reference runs one
+ * query per process). */
+typedef struct swh_server swh_server;
+
+typedef struct swh_server_opts {
+    int max_inflight;          /* requests on the device at once (0 = 64) */
+    uint64_t memory_budget;    /* device bytes for request buffers (0 = 4 GiB) */
+    int slices;                /* database slices per request (0 = 4) */
+    int max_qlen;              /* longest admissible query (0 = 8192) */
+    int max_k;                 /* largest top-k a request may ask for (0 = 100) */
+    swh_search_opts search;    /* strip widths, overflow threshold */
+} swh_server_opts;
+
+typedef struct swh_request_info {
+    double submit_ms, admit_ms, done_ms;   /* server clock: ms since creation */
+    double latency_ms;                     /* done - submit */
+    uint64_t cells;
+    int rounds;
+    int status;                            /* SWH_OK or the failure code */
+} swh_request_info;
+
+typedef struct swh_server_stats {
+    uint64_t rounds, completed, cells;
+    double busy_ms;                        /* wall time spent in rounds */
+    int peak_inflight;
+    int waiting;                           /* submitted, not yet admitted */
+} swh_server_stats;
+
+int swh_server_create(swh_ctx* ctx, swh_db* db, const swh_server_opts* opts, swh_server** out);
+/* Copies the query; returns at once with the request id. */
+int swh_server_submit(swh_server* srv, const uint8_t* query, int qlen, int k, int want_scores,
+                      uint64_t* id);
+/* Blocks until request id is done (SWH_OK) or timeout_ms passes
+ * (SWH_ERR_STATE; timeout_ms < 0 waits forever).  On success writes k
+ * (score, index) pairs, the full score vector if it was asked for at submit
+ * (scores may then be NULL to discard it), and the timing; the request is
+ * then forgotten. */
+int swh_server_wait(swh_server* srv, uint64_t id, int timeout_ms, int32_t* top_score,
+                    int64_t* top_index, int32_t* scores, swh_request_info* info);
+int swh_server_get_stats(swh_server* srv, swh_server_stats* out);
+/* Finishes the requests already admitted, fails the waiting ones, stops. */
+int swh_server_destroy(swh_server* srv);
+
 #ifdef __cplusplus
 }
 #endif

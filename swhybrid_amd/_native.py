@@ -21,6 +21,8 @@ EXPORTS = (
     "swh_last_error", "swh_version", "swh_device_count", "swh_ctx_create",
     "swh_ctx_destroy", "swh_ctx_set_scoring", "swh_db_create", "swh_db_destroy",
     "swh_db_info", "swh_search", "swh_search_batch", "swh_topk",
+    "swh_server_create", "swh_server_submit", "swh_server_wait", "swh_server_get_stats",
+    "swh_server_destroy",
 )
 
 
@@ -48,6 +50,38 @@ class SwhSearchOpts(C.Structure):
         ("group_width", C.c_int),
         ("use_graph", C.c_int),
     ]
+
+
+class SwhServerOpts(C.Structure):
+    _fields_ = [
+        ("max_inflight", C.c_int),
+        ("memory_budget", C.c_uint64),
+        ("slices", C.c_int),
+        ("max_qlen", C.c_int),
+        ("max_k", C.c_int),
+        ("search", SwhSearchOpts),
+    ]
+
+
+class SwhRequestInfo(C.Structure):
+    _fields_ = [
+        ("submit_ms", C.c_double), ("admit_ms", C.c_double), ("done_ms", C.c_double),
+        ("latency_ms", C.c_double), ("cells", C.c_uint64), ("rounds", C.c_int),
+        ("status", C.c_int),
+    ]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class SwhServerStats(C.Structure):
+    _fields_ = [
+        ("rounds", C.c_uint64), ("completed", C.c_uint64), ("cells", C.c_uint64),
+        ("busy_ms", C.c_double), ("peak_inflight", C.c_int), ("waiting", C.c_int),
+    ]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
 
 
 class SwhDbOpts(C.Structure):
@@ -87,6 +121,11 @@ def lib():
     L.swh_search_batch.argtypes = [vp, vp, vp, vp, C.c_int, C.POINTER(SwhSearchOpts), vp,
                                    C.POINTER(SwhStats)]
     L.swh_topk.argtypes = [vp, C.c_uint64, C.c_int, vp, vp]
+    L.swh_server_create.argtypes = [vp, vp, C.POINTER(SwhServerOpts), C.POINTER(vp)]
+    L.swh_server_submit.argtypes = [vp, vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_uint64)]
+    L.swh_server_wait.argtypes = [vp, C.c_uint64, C.c_int, vp, vp, vp, C.POINTER(SwhRequestInfo)]
+    L.swh_server_get_stats.argtypes = [vp, C.POINTER(SwhServerStats)]
+    L.swh_server_destroy.argtypes = [vp]
     for name in EXPORTS:
         if name != "swh_last_error" and name != "swh_version":
             getattr(L, name).restype = C.c_int

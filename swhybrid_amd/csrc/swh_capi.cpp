@@ -1,7 +1,6 @@
 // swh_capi.cpp -- host side of libswh.so: contexts, the resident database
 // layout, query profiles, kernel launches and score copy-back.
-#include "swh.h"
-#include "swh_internal.h"
+#include "swh_host.h"
 
 #include <algorithm>
 #include <chrono>
@@ -13,133 +12,10 @@
 #include <thread>
 #include <vector>
 
-using namespace swh;
-
-namespace {
-
+namespace swhh {
 thread_local std::string g_err;
-
-int fail(int code, const char* fmt, ...) {
-    char buf[512];
-    va_list ap;
-    va_start(ap, fmt);
-    vsnprintf(buf, sizeof(buf), fmt, ap);
-    va_end(ap);
-    g_err = buf;
-    return code;
 }
-
-#define HIP_TRY(expr)                                                                      \
-    do {                                                                                   \
-        hipError_t _e = (expr);                                                            \
-        if (_e != hipSuccess)                                                              \
-            return fail(SWH_ERR_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(_e), \
-                        __FILE__, __LINE__);                                               \
-    } while (0)
-
-double now_ms() {
-    return std::chrono::duration<double, std::milli>(
-               std::chrono::steady_clock::now().time_since_epoch())
-        .count();
-}
-
-template <typename F>
-void parallel_for(int64_t n, F&& f) {
-    unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-    if (n < 4096 || nt == 1) {
-        f(0, n);
-        return;
-    }
-    std::vector<std::thread> ts;
-    int64_t chunk = (n + nt - 1) / nt;
-    for (unsigned t = 0; t < nt; ++t) {
-        int64_t b = t * chunk, e = std::min(n, b + chunk);
-        if (b >= e) break;
-        ts.emplace_back([&f, b, e] { f(b, e); });
-    }
-    for (auto& t : ts) t.join();
-}
-
-// Device buffer that only grows.
-struct DevBuf {
-    void* p = nullptr;
-    size_t bytes = 0;
-    hipError_t reserve(size_t want) {
-        if (want <= bytes) return hipSuccess;
-        if (p) {
-            hipError_t e = hipFree(p);
-            if (e != hipSuccess) return e;
-            p = nullptr;
-            bytes = 0;
-        }
-        hipError_t e = hipMalloc(&p, want);
-        if (e == hipSuccess) bytes = want;
-        return e;
-    }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        bytes = 0;
-    }
-};
-
-struct PinnedBuf {
-    void* p = nullptr;
-    size_t bytes = 0;
-    hipError_t reserve(size_t want) {
-        if (want <= bytes) return hipSuccess;
-        if (p) {
-            (void)hipHostFree(p);
-            p = nullptr;
-            bytes = 0;
-        }
-        hipError_t e = hipHostMalloc(&p, want, hipHostMallocDefault);
-        if (e == hipSuccess) bytes = want;
-        return e;
-    }
-    void release() {
-        if (p) (void)hipHostFree(p);
-        p = nullptr;
-        bytes = 0;
-    }
-};
-
-int check_scoring(const int32_t* matrix, int gap_open, int gap_extend) {
-    if (!matrix) return fail(SWH_ERR_ARG, "matrix is NULL");
-    for (int i = 0; i < 1024; ++i)
-        if (matrix[i] < -128 || matrix[i] > 127)
-            return fail(SWH_ERR_ARG, "matrix entry %d = %d outside [-128, 127]", i, matrix[i]);
-    if (gap_open < 0 || gap_extend < 0 || gap_open + gap_extend > 32767)
-        return fail(SWH_ERR_ARG, "invalid gap penalties open=%d extend=%d", gap_open,
-                    gap_extend);
-    return SWH_OK;
-}
-
-}  // namespace
-
-struct swh_ctx {
-    int device = 0;
-    int num_cus = 0;
-    hipStream_t stream = nullptr;     // every launch and copy of the context
-    int32_t matrix[1024];
-    int gap_open = 10, gap_extend = 2, max_score = 0;
-    DevBuf border, counters, prof, scores, ovf, desc;
-    PinnedBuf h_prof, h_scores, h_desc, h_ctr;
-    hipEvent_t ev[8] = {};
-    size_t border_budget = size_t(8) << 30;   // bytes of HBM for strip borders
-};
-
-struct swh_db {
-    swh_ctx* ctx = nullptr;
-    uint64_t nseq = 0, residues = 0, padded = 0;
-    int n_pairs = 0;            // group-mode pair blocks, blocks [0, n_pairs)
-    int n_blocks = 0;           // all blocks; lane blocks are [n_pairs, n_blocks)
-    uint32_t max_rows_lane = 0, max_rows_pair = 0;
-    uint64_t padded_pair = 0, padded_lane = 0;   // padded residues per layout
-    uint64_t residues_pair = 0;                  // real residues in pair blocks
-    std::vector<BlockDesc> blocks;
-    DevBuf words, d_blocks, perm, sorted_len, sorted_blk;
-};
+using namespace swhh;
 
 extern "C" {
 
@@ -279,6 +155,16 @@ int swh_db_create(swh_ctx* c, const uint8_t* residues, const uint64_t* offsets,
     }
     db->padded = db->padded_pair + db->padded_lane;
     for (uint64_t k = 0; k < lane_first; ++k) db->residues_pair += lengths[order[k]];
+    db->blk_real.assign(db->n_blocks + 1, 0);
+    db->blk_padded.assign(db->n_blocks + 1, 0);
+    for (int b = 0; b < db->n_blocks; ++b) {
+        const BlockDesc& bd = db->blocks[b];
+        const uint64_t n = b < np ? 2 : kLaneSeqs;
+        uint64_t real = 0;
+        for (uint64_t k = bd.first; k < bd.first + n && k < nseq; ++k) real += lengths[order[k]];
+        db->blk_real[b + 1] = db->blk_real[b] + real;
+        db->blk_padded[b + 1] = db->blk_padded[b] + (uint64_t)bd.rows * n;
+    }
     const uint64_t nslots = lane_first + (uint64_t)nl * kLaneSeqs + 2;
     std::vector<uint32_t> words(std::max<uint64_t>(woff, 1));
     std::vector<uint32_t> perm(nslots, kNoSeq);
@@ -369,6 +255,18 @@ int swh_db_info(const swh_db* db, uint64_t* nseq, uint64_t* residues, uint64_t* 
 // most 64); the score is the useful fraction of lane-columns times the share
 // of a lane's step spent on cells (about 13 VALU ops of per-step overhead
 // beside the 9.5 per packed cell).
+}  // extern "C"
+
+namespace swhh {
+
+// Real / padded residues of blocks [b0, b1).
+uint64_t range_residues(const swh_db* db, int b0, int b1) {
+    return b1 > b0 ? db->blk_real[b1] - db->blk_real[b0] : 0;
+}
+uint64_t range_padded(const swh_db* db, int b0, int b1) {
+    return b1 > b0 ? db->blk_padded[b1] - db->blk_padded[b0] : 0;
+}
+
 static double group_eff(int qlen, int W, int req_g, int* G_out) {
     const int qp = ((qlen + W - 1) / W) * W;
     const int strips = qp / W;
@@ -381,51 +279,43 @@ static double group_eff(int qlen, int W, int req_g, int* G_out) {
     return lane_use * (9.0 * W) / (9.5 * W + 13.0) * ((double)qlen / qp);
 }
 
-namespace {
+// Profile row stride of a query: a multiple of 96 covers every strip width
+// (8, 16, 24, 32), so a profile built once serves any launch shape.
+int profile_stride(int qlen) { return ((qlen + 95) / 96) * 96; }
 
-struct Plan {
-    int qlen;
-    int qpad_l, qpad_g, qpad_r, pstride, G;
-    size_t prof_off;   // int16 elements
-};
+// [24][pstride] int16 profile: s(q_j, r) for residue r, zero beyond qlen.
+void build_profile(const int32_t* matrix, const uint8_t* q, int qlen, int16_t* out) {
+    const int ps = profile_stride(qlen);
+    for (int r = 0; r < kProfRows; ++r)
+        for (int j = 0; j < ps; ++j)
+            out[(size_t)r * ps + j] = j < qlen ? (int16_t)matrix[q[j] * 32 + r] : 0;
+}
 
-}  // namespace
-
-// All queries of a call in one launch per kernel: group scan (pair blocks),
-// lane scan (lane blocks), then one int32 rescore launch over every query's
-// overflow list.  scores_host gets nq rows of nseq (may be NULL for no copy).
-static int run_queries(swh_ctx* c, swh_db* db, const uint8_t* const* qs, const int* qlens,
-                       int nq, const swh_search_opts& o, int32_t* scores_host,
-                       swh_stats* stats) {
-    const double t_start = now_ms();
-    swh_stats st{};
-    for (int i = 0; i < nq; ++i) {
-        if (!qs[i]) return fail(SWH_ERR_ARG, "query %d is NULL", i);
-        if (qlens[i] <= 0) return fail(SWH_ERR_ARG, "query %d length %d", i, qlens[i]);
-        for (int j = 0; j < qlens[i]; ++j)
-            if (qs[i][j] > 23)
-                return fail(SWH_ERR_ARG, "query %d code %d at %d > 23", i, qs[i][j], j);
-        st.cells += db->residues * (uint64_t)qlens[i];
-        st.cells_group += db->residues_pair * (uint64_t)qlens[i];
-    }
+int check_opts(const swh_search_opts& o) {
     if (o.group_width && o.group_width != 8 && o.group_width != 16)
         return fail(SWH_ERR_ARG, "group_width must be 8 or 16");
     if (o.group_lanes < 0 || o.group_lanes > kWave)
         return fail(SWH_ERR_ARG, "group_lanes must be in 1..64");
     if (o.strip_width && o.strip_width != 16 && o.strip_width != 24 && o.strip_width != 32)
         return fail(SWH_ERR_ARG, "strip_width must be 16, 24 or 32");
-    HIP_TRY(hipSetDevice(c->device));
-    if (db->nseq == 0 || nq == 0) {
-        st.total_ms = now_ms() - t_start;
-        if (stats) *stats = st;
-        return SWH_OK;
-    }
-    // strip widths shared by the batch.  Lane scan: least padded work, each
-    // width weighted by its measured cost per padded column on MI355X
-    // (5M-sequence database: W=32 is ~1.5% faster than 24 on long queries,
-    // W=16 ~4% slower).  Group scan: W=16 measured best for every length.
-    int Wl = o.strip_width;
-    if (!Wl) {
+    return SWH_OK;
+}
+
+int check_query(const uint8_t* q, int qlen, int i) {
+    if (!q) return fail(SWH_ERR_ARG, "query %d is NULL", i);
+    if (qlen <= 0) return fail(SWH_ERR_ARG, "query %d length %d", i, qlen);
+    for (int j = 0; j < qlen; ++j)
+        if (q[j] > 23) return fail(SWH_ERR_ARG, "query %d code %d at %d > 23", i, q[j], j);
+    return SWH_OK;
+}
+
+// Strip widths shared by a launch.  Lane scan: least padded work, each width
+// weighted by its measured cost per padded column on MI355X (5M-sequence
+// database: W=32 ~1.5% faster than 24 on long queries, W=16 ~4% slower).
+// Group scan: W=16 measured best for every query length.
+Shape choose_shape(const int* qlens, int nq, const swh_search_opts& o) {
+    Shape sh{o.strip_width, o.group_width ? o.group_width : 16};
+    if (!sh.Wl) {
         double best = 1e300;
         for (int w : {24, 32, 16}) {
             const double cost = w == 24 ? 1.0 : (w == 32 ? 0.985 : 1.04);
@@ -433,104 +323,86 @@ static int run_queries(swh_ctx* c, swh_db* db, const uint8_t* const* qs, const i
             for (int i = 0; i < nq; ++i) cols += (double)((qlens[i] + w - 1) / w) * w * cost;
             if (cols < best) {
                 best = cols;
-                Wl = w;
+                sh.Wl = w;
             }
         }
     }
-    int Wg = o.group_width ? o.group_width : 16;
-    std::vector<Plan> plan(nq);
-    size_t prof_elems = 0;
-    int max_qpad_l = 0, max_pass_g = 0, max_gpw = 1;
-    bool lane_border = false, group_border = false;
-    for (int i = 0; i < nq; ++i) {
-        Plan& P = plan[i];
-        P.qlen = qlens[i];
-        P.qpad_l = ((P.qlen + Wl - 1) / Wl) * Wl;
-        P.qpad_g = ((P.qlen + Wg - 1) / Wg) * Wg;
-        P.qpad_r = ((P.qlen + 7) / 8) * 8;
-        group_eff(P.qlen, Wg, o.group_lanes, &P.G);
-        P.pstride = std::max({P.qpad_l, P.qpad_g, P.qpad_r});
-        P.prof_off = prof_elems;
-        prof_elems += (size_t)kProfRows * P.pstride;
-        max_qpad_l = std::max(max_qpad_l, P.qpad_l);
-        max_pass_g = std::max(max_pass_g, std::min(P.G * Wg, P.qpad_g));
-        max_gpw = std::max(max_gpw, kWave / P.G);
-        lane_border |= P.qpad_l > Wl;
-        group_border |= P.qpad_g > P.G * Wg;
-        const int passes = (P.qpad_g / Wg + P.G - 1) / P.G;
-        st.padded_cells += db->padded_lane * (uint64_t)P.qpad_l +
-                           db->padded_pair * (uint64_t)passes * P.G * Wg;
-    }
+    return sh;
+}
 
-    // profiles [24][pstride] int16 per query, zero beyond qlen
-    HIP_TRY(c->h_prof.reserve(prof_elems * sizeof(int16_t)));
-    HIP_TRY(c->prof.reserve(prof_elems * sizeof(int16_t)));
-    int16_t* hp = (int16_t*)c->h_prof.p;
-    for (int i = 0; i < nq; ++i) {
-        const Plan& P = plan[i];
-        int16_t* dst = hp + P.prof_off;
-        for (int r = 0; r < kProfRows; ++r)
-            for (int j = 0; j < P.pstride; ++j)
-                dst[(size_t)r * P.pstride + j] =
-                    j < P.qlen ? (int16_t)c->matrix[qs[i][j] * 32 + r] : 0;
-    }
-    HIP_TRY(hipMemcpyAsync(c->prof.p, hp, prof_elems * sizeof(int16_t), hipMemcpyHostToDevice,
-                           c->stream));
-    const size_t nseq = db->nseq;
-    HIP_TRY(c->scores.reserve((size_t)nq * nseq * sizeof(int32_t)));
-    HIP_TRY(c->ovf.reserve((size_t)nq * nseq * sizeof(uint32_t)));
-    const size_t n_ctr = 16 + (size_t)nq;
-    HIP_TRY(c->counters.reserve(n_ctr * sizeof(uint32_t)));
-    uint32_t* ctr = (uint32_t*)c->counters.p;   // [0] scan, [2] rescore; [16+q] overflow count
-    HIP_TRY(launch_reset(ctr, (int)n_ctr, c->stream));
+// Enqueue, without waiting: counter reset, one scan launch for every task and
+// the int32 rescore of their overflow lists.  counters needs 16 + nt words;
+// counters[16 + i] ends up holding task i's overflow count.
+int enqueue_tasks(swh_ctx* c, swh_db* db, const Task* t, int nt, const Shape& sh,
+                         const swh_search_opts& o, uint32_t* counters, swh_stats* st) {
     const int exact_thr = 65535 - c->max_score;
     const int thr = (o.overflow_threshold > 0) ? std::min(o.overflow_threshold, exact_thr)
                                                : exact_thr;
     const int gap_oe = c->gap_open + c->gap_extend;
-    const int n_lane = db->n_blocks - db->n_pairs;
-
-    // query tables: lane, group, rescore
-    HIP_TRY(c->h_desc.reserve(3 * nq * sizeof(QueryDesc)));
-    HIP_TRY(c->desc.reserve(3 * nq * sizeof(QueryDesc)));
+    HIP_TRY(c->h_desc.reserve(3 * (size_t)nt * sizeof(QueryDesc)));
+    HIP_TRY(c->desc.reserve(3 * (size_t)nt * sizeof(QueryDesc)));
     QueryDesc* hd = (QueryDesc*)c->h_desc.p;
-    int items_l = 0, items_g = 0;
-    for (int i = 0; i < nq; ++i) {
+    int items_l = 0, items_g = 0, max_qpad_l = 0, max_pass_g = 0, max_gpw = 1, max_qpad_r = 0;
+    bool lane_border = false, group_border = false;
+    for (int i = 0; i < nt; ++i) {
+        const Task& T = t[i];
         QueryDesc base{};
-        base.prof = (const int16_t*)c->prof.p + plan[i].prof_off;
-        base.scores = (int32_t*)c->scores.p + (size_t)i * nseq;
-        base.ovf_list = (uint32_t*)c->ovf.p + (size_t)i * nseq;
-        base.ovf_count = ctr + 16 + i;
-        base.qlen = plan[i].qlen;
-        base.pstride = plan[i].pstride;
+        base.prof = T.prof;
+        base.scores = T.scores;
+        base.ovf_list = T.ovf;
+        base.ovf_count = counters + 16 + i;
+        base.qlen = T.qlen;
+        base.pstride = profile_stride(T.qlen);
         base.ovf_threshold = thr;
         QueryDesc& L = hd[i];
         L = base;
-        L.qpad = plan[i].qpad_l;
+        L.qpad = ((T.qlen + sh.Wl - 1) / sh.Wl) * sh.Wl;
         L.group = 1;
+        L.blk_begin = T.lane_begin;
+        L.blk_end = T.lane_end;
         L.item_begin = items_l;
-        L.n_items = (n_lane + kWavesPerWG - 1) / kWavesPerWG;
+        L.n_items = (std::max(0, T.lane_end - T.lane_begin) + kWavesPerWG - 1) / kWavesPerWG;
         items_l += L.n_items;
-        QueryDesc& Gd = hd[nq + i];
+        max_qpad_l = std::max(max_qpad_l, L.qpad);
+        lane_border |= L.n_items > 0 && L.qpad > sh.Wl;
+        QueryDesc& Gd = hd[nt + i];
         Gd = base;
-        Gd.qpad = plan[i].qpad_g;
-        Gd.group = plan[i].G;
+        Gd.qpad = ((T.qlen + sh.Wg - 1) / sh.Wg) * sh.Wg;
+        group_eff(T.qlen, sh.Wg, o.group_lanes, &Gd.group);
+        Gd.blk_begin = T.pair_begin;
+        Gd.blk_end = T.pair_end;
         Gd.item_begin = items_g;
-        const int per_item = kWavesPerWG * (kWave / plan[i].G);
-        Gd.n_items = (db->n_pairs + per_item - 1) / per_item;
+        const int per_item = kWavesPerWG * (kWave / Gd.group);
+        Gd.n_items = (std::max(0, T.pair_end - T.pair_begin) + per_item - 1) / per_item;
         items_g += Gd.n_items;
-        QueryDesc& Rd = hd[2 * nq + i];
+        if (Gd.n_items > 0) {
+            max_pass_g = std::max(max_pass_g, std::min(Gd.group * sh.Wg, Gd.qpad));
+            max_gpw = std::max(max_gpw, kWave / Gd.group);
+            group_border |= Gd.qpad > Gd.group * sh.Wg;
+        }
+        QueryDesc& Rd = hd[2 * nt + i];
         Rd = base;
-        Rd.qpad = plan[i].qpad_r;
+        Rd.qpad = ((T.qlen + 7) / 8) * 8;
         Rd.group = 1;
+        max_qpad_r = std::max(max_qpad_r, Rd.qpad);
+        st->cells += (uint64_t)T.qlen *
+                     (range_residues(db, T.pair_begin, T.pair_end) +
+                      range_residues(db, T.lane_begin, T.lane_end));
+        st->cells_group += (uint64_t)T.qlen * range_residues(db, T.pair_begin, T.pair_end);
+        const int passes = (Gd.qpad / sh.Wg + Gd.group - 1) / Gd.group;
+        st->padded_cells +=
+            range_padded(db, T.lane_begin, T.lane_end) * (uint64_t)L.qpad +
+            range_padded(db, T.pair_begin, T.pair_end) * (uint64_t)passes * Gd.group * sh.Wg;
     }
-    HIP_TRY(hipMemcpyAsync(c->desc.p, hd, 2 * nq * sizeof(QueryDesc), hipMemcpyHostToDevice,
-                           c->stream));
+    HIP_TRY(hipMemcpyAsync(c->desc.p, hd, 3 * (size_t)nt * sizeof(QueryDesc),
+                           hipMemcpyHostToDevice, c->stream));
     const QueryDesc* d_lane = (const QueryDesc*)c->desc.p;
-    const QueryDesc* d_group = d_lane + nq;
-    const QueryDesc* d_res = d_lane + 2 * nq;
+    const QueryDesc* d_group = d_lane + nt;
+    const QueryDesc* d_res = d_lane + 2 * nt;
+    HIP_TRY(launch_reset(counters, 16 + nt, c->stream));
 
-    // one persistent launch: group items (long pairs) first, then lane items
-    const int ps_l = std::min(max_qpad_l, kSegCols) + kProfPad;
+    // scan: group items (long pairs) first, then lane items, one launch
+    const int ps_l = std::min(std::max(max_qpad_l, 8), kSegCols) + kProfPad;
     const int ps_g = max_pass_g + kProfPad;
     const size_t lds = (size_t)kProfRows * std::max(ps_l, ps_g) * sizeof(int16_t);
     const uint64_t stride_l = lane_border ? (uint64_t)db->max_rows_lane * kWave : 0;
@@ -538,11 +410,20 @@ static int run_queries(swh_ctx* c, swh_db* db, const uint8_t* const* qs, const i
     const uint64_t stride = std::max(stride_l, stride_g);
     const uint64_t per_wg = stride * sizeof(uint2) * kWavesPerWG;
     const int items = items_g + items_l;
-    int grid = c->num_cus * scan_max_wg_per_cu(Wl, Wg, lds);
+    int grid = c->num_cus * scan_max_wg_per_cu(sh.Wl, sh.Wg, lds);
     if (per_wg)
         grid = (int)std::min<uint64_t>(grid, std::max<uint64_t>(1, c->border_budget / per_wg));
     grid = std::max(1, std::min(grid, items));
-    HIP_TRY(c->border.reserve(std::max<uint64_t>(16, per_wg * (uint64_t)grid)));
+    // rescore: one sequence per lane, grid capped so its borders fit 1 GiB
+    const int ps_r = std::min(max_qpad_r, kSegCols) + kProfPad;
+    const size_t lds_r = (size_t)kProfRows * ps_r * sizeof(int16_t);
+    const uint32_t lmax = std::max(db->max_rows_lane, db->max_rows_pair);
+    const uint64_t rstride = (uint64_t)lmax * kWave;
+    const uint64_t r_per_wg = rstride * sizeof(uint2) * kWavesPerWG;
+    int rgrid = c->num_cus * rescore_max_wg_per_cu(lds_r);
+    rgrid = (int)std::max<uint64_t>(1, std::min<uint64_t>(rgrid, (size_t(1) << 30) / r_per_wg));
+    HIP_TRY(c->border.reserve(
+        std::max<uint64_t>({16, per_wg * (uint64_t)grid, r_per_wg * (uint64_t)rgrid})));
 
     ScanArgs a{};
     a.words = (const uint32_t*)db->words.p;
@@ -550,100 +431,118 @@ static int run_queries(swh_ctx* c, swh_db* db, const uint8_t* const* qs, const i
     a.perm = (const uint32_t*)db->perm.p;
     a.group_q = d_group;
     a.lane_q = d_lane;
-    a.work_counter = ctr + 0;
+    a.work_counter = counters + 0;
     a.border = (uint2*)c->border.p;
     a.border_stride = stride;
     a.border_rows = db->max_rows_pair;
-    a.n_queries = nq;
-    a.n_items_group = db->n_pairs > 0 ? items_g : 0;
-    a.n_items_lane = n_lane > 0 ? items_l : 0;
-    a.pair_end = db->n_pairs;
-    a.lane_begin = db->n_pairs;
-    a.lane_end = db->n_blocks;
+    a.n_queries = nt;
+    a.n_items_group = items_g;
+    a.n_items_lane = items_l;
     a.ps_lane = ps_l;
     a.ps_group = ps_g;
     a.gap_oe = gap_oe;
     a.gap_e = c->gap_extend;
-    HIP_TRY(hipEventRecord(c->ev[0], c->stream));
-    if (a.n_items_group + a.n_items_lane > 0) {
-        HIP_TRY(launch_scan(Wl, Wg, a, grid, lds, c->stream));
-        st.n_launches += 1;
+    if (items > 0) {
+        HIP_TRY(launch_scan(sh.Wl, sh.Wg, a, grid, lds, c->stream));
+        st->n_launches += 1;
     }
-    HIP_TRY(hipEventRecord(c->ev[1], c->stream));
+    RescoreArgs r{};
+    r.words = a.words;
+    r.blocks = a.blocks;
+    r.perm = a.perm;
+    r.sorted_len = (const uint32_t*)db->sorted_len.p;
+    r.sorted_blk = (const uint32_t*)db->sorted_blk.p;
+    r.queries = d_res;
+    r.work_counter = counters + 2;
+    r.border = (uint2*)c->border.p;
+    r.border_stride = rstride;
+    r.n_queries = nt;
+    r.n_lane_blocks_first = db->n_pairs;
+    r.ps = ps_r;
+    r.gap_oe = gap_oe;
+    r.gap_e = c->gap_extend;
+    HIP_TRY(launch_rescore(r, rgrid, lds_r, c->stream));
+    st->n_launches += 1;
+    return SWH_OK;
+}
 
-    // overflow counts -> rescore work items
-    HIP_TRY(c->h_ctr.reserve(n_ctr * sizeof(uint32_t)));
-    uint32_t* hc = (uint32_t*)c->h_ctr.p;
-    HIP_TRY(hipMemcpyAsync(hc, ctr, n_ctr * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    int items_r = 0;
+// Search every query over the whole database in one launch sequence.
+// scores_host gets nq rows of nseq.
+static int run_queries(swh_ctx* c, swh_db* db, const uint8_t* const* qs, const int* qlens,
+                       int nq, const swh_search_opts& o, int32_t* scores_host,
+                       swh_stats* stats) {
+    const double t_start = now_ms();
+    swh_stats st{};
+    int rc = check_opts(o);
+    if (rc) return rc;
+    for (int i = 0; i < nq; ++i)
+        if ((rc = check_query(qs[i], qlens[i], i))) return rc;
+    HIP_TRY(hipSetDevice(c->device));
+    if (db->nseq == 0 || nq == 0) {
+        for (int i = 0; i < nq; ++i) st.cells += 0;
+        st.total_ms = now_ms() - t_start;
+        if (stats) *stats = st;
+        return SWH_OK;
+    }
+    const Shape sh = choose_shape(qlens, nq, o);
+    std::vector<size_t> prof_off(nq);
+    size_t prof_elems = 0;
     for (int i = 0; i < nq; ++i) {
-        QueryDesc& Rd = hd[2 * nq + i];
-        Rd.item_begin = items_r;
-        Rd.n_items = (int)((hc[16 + i] + 255) / 256);
-        items_r += Rd.n_items;
-        st.n_overflow += hc[16 + i];
+        prof_off[i] = prof_elems;
+        prof_elems += (size_t)kProfRows * profile_stride(qlens[i]);
     }
-    if (items_r > 0) {
-        HIP_TRY(hipMemcpyAsync((void*)d_res, hd + 2 * nq, nq * sizeof(QueryDesc),
-                               hipMemcpyHostToDevice, c->stream));
-        RescoreArgs r{};
-        r.words = a.words;
-        r.blocks = a.blocks;
-        r.perm = a.perm;
-        r.sorted_len = (const uint32_t*)db->sorted_len.p;
-        r.sorted_blk = (const uint32_t*)db->sorted_blk.p;
-        r.queries = d_res;
-        r.work_counter = ctr + 2;
-        r.n_queries = nq;
-        r.n_items = items_r;
-        r.n_lane_blocks_first = db->n_pairs;
-        int max_qpad_r = 0;
-        for (const Plan& P : plan) max_qpad_r = std::max(max_qpad_r, P.qpad_r);
-        r.ps = std::min(max_qpad_r, kSegCols) + kProfPad;
-        r.gap_oe = gap_oe;
-        r.gap_e = c->gap_extend;
-        const size_t lds_r = (size_t)kProfRows * r.ps * sizeof(int16_t);
-        const uint32_t lmax = std::max(db->max_rows_lane, db->max_rows_pair);
-        const uint64_t rstride = (uint64_t)lmax * kWave;
-        const uint64_t r_per_wg = rstride * sizeof(uint2) * kWavesPerWG;
-        int rgrid = (int)std::min<uint64_t>((uint64_t)items_r,
-                                            (uint64_t)c->num_cus * rescore_max_wg_per_cu(lds_r));
-        rgrid = (int)std::max<uint64_t>(1, std::min<uint64_t>(rgrid, c->border_budget / r_per_wg));
-        HIP_TRY(c->border.reserve(r_per_wg * rgrid));
-        r.border = (uint2*)c->border.p;
-        r.border_stride = rstride;
-        HIP_TRY(hipEventRecord(c->ev[2], c->stream));
-        HIP_TRY(launch_rescore(r, rgrid, lds_r, c->stream));
-        HIP_TRY(hipEventRecord(c->ev[3], c->stream));
-        st.n_launches += 1;
+    const size_t nseq = db->nseq;
+    const size_t score_bytes = (size_t)nq * nseq * sizeof(int32_t);
+    HIP_TRY(c->h_prof.reserve(prof_elems * sizeof(int16_t)));
+    HIP_TRY(c->prof.reserve(prof_elems * sizeof(int16_t)));
+    HIP_TRY(c->scores.reserve(score_bytes));
+    HIP_TRY(c->ovf.reserve((size_t)nq * nseq * sizeof(uint32_t)));
+    HIP_TRY(c->counters.reserve((16 + (size_t)nq) * sizeof(uint32_t)));
+    HIP_TRY(c->h_ctr.reserve((16 + (size_t)nq) * sizeof(uint32_t)));
+    const bool pinned_out = score_bytes <= (size_t(64) << 20);
+    if (pinned_out) HIP_TRY(c->h_scores.reserve(score_bytes));
+    int16_t* hp = (int16_t*)c->h_prof.p;
+    std::vector<Task> tasks(nq);
+    for (int i = 0; i < nq; ++i) {
+        build_profile(c->matrix, qs[i], qlens[i], hp + prof_off[i]);
+        Task& T = tasks[i];
+        T.prof = (const int16_t*)c->prof.p + prof_off[i];
+        T.qlen = qlens[i];
+        T.scores = (int32_t*)c->scores.p + (size_t)i * nseq;
+        T.ovf = (uint32_t*)c->ovf.p + (size_t)i * nseq;
+        T.pair_begin = 0;
+        T.pair_end = db->n_pairs;
+        T.lane_begin = db->n_pairs;
+        T.lane_end = db->n_blocks;
     }
-    if (scores_host) {
-        const size_t bytes = (size_t)nq * nseq * sizeof(int32_t);
-        if (bytes <= (size_t(64) << 20)) {
-            HIP_TRY(c->h_scores.reserve(bytes));
-            HIP_TRY(hipMemcpyAsync(c->h_scores.p, c->scores.p, bytes, hipMemcpyDeviceToHost,
-                                   c->stream));
-            HIP_TRY(hipStreamSynchronize(c->stream));
-            std::memcpy(scores_host, c->h_scores.p, bytes);
-        } else {
-            HIP_TRY(hipStreamSynchronize(c->stream));
-            HIP_TRY(hipMemcpy(scores_host, c->scores.p, bytes, hipMemcpyDeviceToHost));
-        }
-    } else {
-        HIP_TRY(hipStreamSynchronize(c->stream));
-    }
+    uint32_t* ctr = (uint32_t*)c->counters.p;
+    HIP_TRY(hipMemcpyAsync(c->prof.p, hp, prof_elems * sizeof(int16_t), hipMemcpyHostToDevice,
+                           c->stream));
+    HIP_TRY(hipEventRecord(c->ev[0], c->stream));
+    rc = enqueue_tasks(c, db, tasks.data(), nq, sh, o, ctr, &st);
+    if (rc) return rc;
+    HIP_TRY(hipEventRecord(c->ev[1], c->stream));
+    HIP_TRY(hipMemcpyAsync(c->h_ctr.p, ctr, (16 + (size_t)nq) * sizeof(uint32_t),
+                           hipMemcpyDeviceToHost, c->stream));
+    if (pinned_out)
+        HIP_TRY(hipMemcpyAsync(c->h_scores.p, c->scores.p, score_bytes, hipMemcpyDeviceToHost,
+                               c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (pinned_out) std::memcpy(scores_host, c->h_scores.p, score_bytes);
+    else HIP_TRY(hipMemcpy(scores_host, c->scores.p, score_bytes, hipMemcpyDeviceToHost));
+    const uint32_t* hc = (const uint32_t*)c->h_ctr.p;
+    for (int i = 0; i < nq; ++i) st.n_overflow += hc[16 + i];
     float ms = 0;
     HIP_TRY(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
-    st.kernel_ms = ms;
-    if (items_r > 0) {
-        HIP_TRY(hipEventElapsedTime(&ms, c->ev[2], c->ev[3]));
-        st.rescore_ms = ms;
-    }
+    st.kernel_ms = ms;   // scan + rescore
     st.total_ms = now_ms() - t_start;
     if (stats) *stats = st;
     return SWH_OK;
 }
+
+}  // namespace swhh
+
+extern "C" {
 
 int swh_search(swh_ctx* c, swh_db* db, const uint8_t* query, int qlen,
                const swh_search_opts* opts, int32_t* scores, swh_stats* stats) {

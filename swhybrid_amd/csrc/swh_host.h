@@ -1,0 +1,182 @@
+// swh_host.h -- private host-side definitions shared by the sources of
+// libswh.so (contexts, the resident database, launch planning).  Not part of
+// the public ABI (include/swh.h).
+#pragma once
+#include <algorithm>
+#include <chrono>
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "swh.h"
+#include "swh_internal.h"
+
+namespace swhh {
+
+using namespace swh;
+
+extern thread_local std::string g_err;
+
+inline int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                      \
+    do {                                                                                   \
+        hipError_t _e = (expr);                                                            \
+        if (_e != hipSuccess)                                                              \
+            return ::swhh::fail(SWH_ERR_HIP, "%s failed: %s (%s:%d)", #expr,               \
+                                hipGetErrorString(_e), __FILE__, __LINE__);                \
+    } while (0)
+
+inline double now_ms() {
+    return std::chrono::duration<double, std::milli>(
+               std::chrono::steady_clock::now().time_since_epoch())
+        .count();
+}
+
+template <typename F>
+inline void parallel_for(int64_t n, F&& f) {
+    unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    if (n < 4096 || nt == 1) {
+        f(0, n);
+        return;
+    }
+    std::vector<std::thread> ts;
+    int64_t chunk = (n + nt - 1) / nt;
+    for (unsigned t = 0; t < nt; ++t) {
+        int64_t b = t * chunk, e = std::min(n, b + chunk);
+        if (b >= e) break;
+        ts.emplace_back([&f, b, e] { f(b, e); });
+    }
+    for (auto& t : ts) t.join();
+}
+
+// Device buffer that only grows.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    hipError_t reserve(size_t want) {
+        if (want <= bytes) return hipSuccess;
+        if (p) {
+            hipError_t e = hipFree(p);
+            if (e != hipSuccess) return e;
+            p = nullptr;
+            bytes = 0;
+        }
+        hipError_t e = hipMalloc(&p, want);
+        if (e == hipSuccess) bytes = want;
+        return e;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+};
+
+struct PinnedBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    hipError_t reserve(size_t want) {
+        if (want <= bytes) return hipSuccess;
+        if (p) {
+            (void)hipHostFree(p);
+            p = nullptr;
+            bytes = 0;
+        }
+        hipError_t e = hipHostMalloc(&p, want, hipHostMallocDefault);
+        if (e == hipSuccess) bytes = want;
+        return e;
+    }
+    void release() {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+};
+
+inline int check_scoring(const int32_t* matrix, int gap_open, int gap_extend) {
+    if (!matrix) return fail(SWH_ERR_ARG, "matrix is NULL");
+    for (int i = 0; i < 1024; ++i)
+        if (matrix[i] < -128 || matrix[i] > 127)
+            return fail(SWH_ERR_ARG, "matrix entry %d = %d outside [-128, 127]", i, matrix[i]);
+    if (gap_open < 0 || gap_extend < 0 || gap_open + gap_extend > 32767)
+        return fail(SWH_ERR_ARG, "invalid gap penalties open=%d extend=%d", gap_open,
+                    gap_extend);
+    return SWH_OK;
+}
+
+
+}  // namespace swhh
+
+
+using swhh::DevBuf;
+using swhh::PinnedBuf;
+using swh::BlockDesc;
+
+struct swh_ctx {
+    int device = 0;
+    int num_cus = 0;
+    hipStream_t stream = nullptr;     // every launch and copy of the context
+    int32_t matrix[1024];
+    int gap_open = 10, gap_extend = 2, max_score = 0;
+    DevBuf border, counters, prof, scores, ovf, desc;
+    PinnedBuf h_prof, h_scores, h_desc, h_ctr;
+    hipEvent_t ev[8] = {};
+    size_t border_budget = size_t(8) << 30;   // bytes of HBM for strip borders
+};
+
+struct swh_db {
+    swh_ctx* ctx = nullptr;
+    std::vector<uint64_t> blk_real;     // prefix sums over blocks of real residues
+    std::vector<uint64_t> blk_padded;   // ... and of padded residues (rows x sequences)
+    uint64_t nseq = 0, residues = 0, padded = 0;
+    int n_pairs = 0;            // group-mode pair blocks, blocks [0, n_pairs)
+    int n_blocks = 0;           // all blocks; lane blocks are [n_pairs, n_blocks)
+    uint32_t max_rows_lane = 0, max_rows_pair = 0;
+    uint64_t padded_pair = 0, padded_lane = 0;   // padded residues per layout
+    uint64_t residues_pair = 0;                  // real residues in pair blocks
+    std::vector<BlockDesc> blocks;
+    DevBuf words, d_blocks, perm, sorted_len, sorted_blk;
+};
+
+
+namespace swhh {
+
+// One query of a launch, with device buffers that outlive the launch.
+struct Task {
+    const int16_t* prof;        // device profile [24][profile_stride(qlen)]
+    int qlen;
+    int32_t* scores;            // device, caller order (nseq entries)
+    uint32_t* ovf;              // device overflow list, >= sequences in the ranges
+    int pair_begin, pair_end;   // pair blocks scanned by this task
+    int lane_begin, lane_end;   // lane blocks scanned by this task
+};
+
+struct Shape {
+    int Wl, Wg;
+};
+
+
+int profile_stride(int qlen);
+void build_profile(const int32_t* matrix, const uint8_t* q, int qlen, int16_t* out);
+uint64_t range_residues(const swh_db* db, int b0, int b1);
+uint64_t range_padded(const swh_db* db, int b0, int b1);
+int check_opts(const swh_search_opts& o);
+int check_query(const uint8_t* q, int qlen, int i);
+Shape choose_shape(const int* qlens, int nq, const swh_search_opts& o);
+int enqueue_tasks(swh_ctx* c, swh_db* db, const Task* t, int nt, const Shape& sh,
+                  const swh_search_opts& o, uint32_t* counters, swh_stats* st);
+
+}  // namespace swhh

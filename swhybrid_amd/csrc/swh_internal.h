@@ -41,7 +41,8 @@ struct QueryDesc {
     int item_begin;
     int n_items;
     int ovf_threshold;       // score >= threshold -> int32 rescore
-    int pad_;
+    int blk_begin, blk_end;  // blocks of this query (pair blocks in the group
+                             // table, lane blocks in the lane table)
 };
 
 struct ScanArgs {
@@ -56,8 +57,6 @@ struct ScanArgs {
     uint32_t border_rows;      // group items: rows reserved per group (longest pair)
     int n_queries;
     int n_items_group, n_items_lane;
-    int pair_end;              // pair blocks are [0, pair_end)
-    int lane_begin, lane_end;  // lane blocks
     int ps_lane, ps_group;     // LDS profile row strides (int16)
     int gap_oe, gap_e;
 };
@@ -76,12 +75,11 @@ struct RescoreArgs {
     const uint32_t* perm;
     const uint32_t* sorted_len;   // real length per sorted position
     const uint32_t* sorted_blk;   // block of each sorted position
-    const QueryDesc* queries;     // qpad = qlen rounded to 8; n_items from the overflow counts
+    const QueryDesc* queries;     // qpad = qlen rounded to 8; items from the overflow counts
     uint32_t* work_counter;
     uint2* border;
     uint64_t border_stride;
     int n_queries;
-    int n_items;
     int n_lane_blocks_first;      // index of the first lane-mode block
     int ps;
     int gap_oe, gap_e;

@@ -222,30 +222,6 @@ __device__ __forceinline__ void emit(const QueryDesc& Qd, const uint32_t* perm, 
     }
 }
 
-// Thread 0 claims the next work item and finds its query (the last query whose
-// first item is <= the claimed one); the workgroup reads both from LDS.
-__device__ __forceinline__ bool next_item(uint32_t* counter, const QueryDesc* qs, int nq,
-                                          int n_items, int* s_buf, int& item, int& q) {
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const int it = (int)atomicAdd(counter, 1u);
-        int lo = 0, hi = nq - 1;
-        if (it < n_items) {
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if (qs[mid].item_begin <= it) lo = mid;
-                else hi = mid - 1;
-            }
-        }
-        s_buf[0] = it;
-        s_buf[1] = lo;
-    }
-    __syncthreads();
-    item = s_buf[0];
-    q = s_buf[1];
-    return item < n_items;
-}
-
 // One lane-mode work item: 4 consecutive lane blocks of query q, one per wave.
 template <int W>
 __device__ __forceinline__ void lane_item(const ScanArgs& A, const QueryDesc& Qd, int q,
@@ -255,8 +231,8 @@ __device__ __forceinline__ void lane_item(const ScanArgs& A, const QueryDesc& Qd
     const int lane = threadIdx.x & 63;
     const s2 Q = {(short)A.gap_oe, (short)A.gap_oe};
     const s2 R = {(short)A.gap_e, (short)A.gap_e};
-    const int blk = A.lane_begin + local * kWavesPerWG + wave;
-    const bool active = blk < A.lane_end;
+    const int blk = Qd.blk_begin + local * kWavesPerWG + wave;
+    const bool active = blk < Qd.blk_end;
     BlockDesc bd = {0, 0, 0};
     if (active) bd = A.blocks[blk];
     const uint2* wds = reinterpret_cast<const uint2*>(A.words + bd.word_off);
@@ -311,8 +287,8 @@ __device__ __forceinline__ void group_item(const ScanArgs& A, const QueryDesc& Q
     const int gl = lane - gi * G;
     const bool in_group = gi < gpw;
     uint2* bord = slot + (size_t)(in_group ? gi : 0) * A.border_rows;
-    const int p = (local * kWavesPerWG + wave) * gpw + gi;
-    const bool has_pair = in_group && p < A.pair_end;
+    const int p = Qd.blk_begin + (local * kWavesPerWG + wave) * gpw + gi;
+    const bool has_pair = in_group && p < Qd.blk_end;
     BlockDesc bd = {0, 0, 0};
     if (has_pair) bd = A.blocks[p];
     uint32_t rmax = bd.rows;
@@ -470,6 +446,32 @@ __device__ __forceinline__ void single_profile(int (&P)[W], const short* lds, in
     for (int c = 0; c < W; ++c) P[c] = pa[c];
 }
 
+// Rescore work items are 256 overflow-list entries; query q owns
+// ceil(count_q / 256) of them.  Thread 0 walks the (short) query table to map
+// a claimed item to its query, so no host round trip sizes this launch.
+__device__ __forceinline__ bool next_rescore_item(const RescoreArgs& A, int* s_buf, int& local,
+                                                  int& q) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int it = (int)atomicAdd(A.work_counter, 1u);
+        int found = -1;
+        for (int j = 0; j < A.n_queries; ++j) {
+            const int n = (int)((*A.queries[j].ovf_count + 255u) >> 8);
+            if (it < n) {
+                found = j;
+                break;
+            }
+            it -= n;
+        }
+        s_buf[0] = found >= 0 ? it : -1;
+        s_buf[1] = found;
+    }
+    __syncthreads();
+    local = s_buf[0];
+    q = s_buf[1];
+    return local >= 0;
+}
+
 __global__ __launch_bounds__(256) void k_rescore(RescoreArgs A) {
     constexpr int W = 8;   // divides every padded query length (all are multiples of 8)
     extern __shared__ __align__(16) short lds[];
@@ -478,11 +480,11 @@ __global__ __launch_bounds__(256) void k_rescore(RescoreArgs A) {
     const int lane = threadIdx.x & 63;
     uint2* bord = A.border + (size_t)(blockIdx.x * kWavesPerWG + wave) * A.border_stride;
     int loaded_q = -1, loaded_seg = -1;
-    int item, q;
-    while (next_item(A.work_counter, A.queries, A.n_queries, A.n_items, s_buf, item, q)) {
+    int local, q;
+    while (next_rescore_item(A, s_buf, local, q)) {
         const QueryDesc Qd = A.queries[q];
         const int n = (int)*Qd.ovf_count;
-        const int entry = (item - Qd.item_begin) * 256 + threadIdx.x;
+        const int entry = local * 256 + threadIdx.x;
         const bool valid = entry < n;
         uint32_t s = 0, len = 0;
         uint64_t base = 0;

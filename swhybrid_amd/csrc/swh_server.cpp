@@ -1,0 +1,398 @@
+// swh_server.cpp -- continuous-batching search service on a resident database.
+//
+// A loop thread repeats rounds.  Each round it admits waiting requests (in
+// arrival order) while their device buffers fit the memory budget and a slot
+// is free, then enqueues ONE scan launch (plus the rescore) covering the next
+// database slice of every admitted request, waits for it, and hands the
+// requests that finished their last slice to a completion thread (top-k on the
+// host from the pinned copy of the score vector).  Scores of a request depend
+// only on its query and the database: each task scans its own block ranges and
+// writes its own buffers, whatever else shares the launch.
+#include <condition_variable>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <unordered_map>
+
+#include "swh_host.h"
+
+using namespace swhh;
+
+namespace {
+
+struct Request {
+    uint64_t id = 0;
+    std::vector<uint8_t> query;
+    int k = 0;
+    bool want_scores = false;
+    int slot = -1;
+    int next_slice = 0;
+    size_t bytes = 0;                 // device bytes charged against the budget
+    swh_request_info info{};
+    std::vector<int32_t> top_score;
+    std::vector<int64_t> top_index;
+    std::vector<int32_t> scores;      // copy of the score vector if asked for
+    bool done = false;
+};
+
+}  // namespace
+
+struct swh_server {
+    swh_ctx* ctx = nullptr;
+    swh_db* db = nullptr;
+    swh_server_opts opts{};
+    double t0 = 0;
+    // slices of the block ranges
+    std::vector<int> pair_cut, lane_cut;
+    size_t max_slice_seqs = 0;
+    // per-slot device buffers and pinned score copies
+    DevBuf scores, ovf, prof, counters;
+    PinnedBuf h_prof, h_scores;
+    size_t prof_slot_elems = 0;
+    std::vector<int> free_slots;
+    size_t bytes_in_use = 0;
+    // request state
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done, cv_complete;
+    std::deque<std::shared_ptr<Request>> waiting;
+    std::vector<std::shared_ptr<Request>> active;
+    std::deque<std::shared_ptr<Request>> to_complete;
+    std::unordered_map<uint64_t, std::shared_ptr<Request>> all;
+    uint64_t next_id = 1;
+    bool stop = false;        // no new admissions; the loop ends once nothing is active
+    bool loop_done = false;   // the loop thread has returned
+    std::string loop_error;
+    swh_server_stats stats{};
+    std::thread loop, completer;
+};
+
+namespace {
+
+double since(const swh_server* s) { return now_ms() - s->t0; }
+
+size_t request_bytes(const swh_server* s, int qlen) {
+    return s->db->nseq * sizeof(int32_t) + s->max_slice_seqs * sizeof(uint32_t) +
+           (size_t)kProfRows * profile_stride(qlen) * sizeof(int16_t);
+}
+
+// Split [b0, b1) into n parts of about equal padded work.
+std::vector<int> cut_blocks(const swh_db* db, int b0, int b1, int n) {
+    std::vector<int> cut(n + 1, b0);
+    cut[n] = b1;
+    const uint64_t base = db->blk_padded[b0];
+    const uint64_t total = db->blk_padded[b1] - base;
+    int b = b0;
+    for (int i = 1; i < n; ++i) {
+        const uint64_t target = base + total * (uint64_t)i / n;
+        while (b < b1 && db->blk_padded[b + 1] <= target) ++b;
+        cut[i] = std::max(b, cut[i - 1]);
+    }
+    return cut;
+}
+
+// One round: admit, launch the next slice of every active request, wait.
+// Returns false when the loop should end.
+bool run_round(swh_server* s) {
+    swh_ctx* c = s->ctx;
+    swh_db* db = s->db;
+    std::vector<std::shared_ptr<Request>> batch;
+    std::vector<std::shared_ptr<Request>> admitted;
+    {
+        std::unique_lock<std::mutex> lk(s->mu);
+        s->cv_work.wait(lk, [&] { return s->stop || !s->waiting.empty() || !s->active.empty(); });
+        while (!s->stop && !s->waiting.empty() && !s->free_slots.empty()) {
+            auto& r = s->waiting.front();
+            if (s->bytes_in_use + r->bytes > s->opts.memory_budget && !s->active.empty()) break;
+            r->slot = s->free_slots.back();
+            s->free_slots.pop_back();
+            s->bytes_in_use += r->bytes;
+            r->info.admit_ms = since(s);
+            s->active.push_back(r);
+            admitted.push_back(r);
+            s->waiting.pop_front();
+        }
+        if (s->active.empty()) return !s->stop;
+        batch = s->active;
+        s->stats.peak_inflight = std::max<int>(s->stats.peak_inflight, (int)batch.size());
+        s->stats.waiting = (int)s->waiting.size();
+    }
+    const double t_round = now_ms();
+    (void)hipSetDevice(c->device);
+    // profiles of newly admitted requests
+    for (auto& r : admitted) {
+        int16_t* hp = (int16_t*)s->h_prof.p + (size_t)r->slot * s->prof_slot_elems;
+        build_profile(c->matrix, r->query.data(), (int)r->query.size(), hp);
+        const size_t n = (size_t)kProfRows * profile_stride((int)r->query.size());
+        if (hipMemcpyAsync((int16_t*)s->prof.p + (size_t)r->slot * s->prof_slot_elems, hp,
+                           n * sizeof(int16_t), hipMemcpyHostToDevice, c->stream) != hipSuccess) {
+            s->loop_error = "profile upload failed";
+            return false;
+        }
+    }
+    std::vector<Task> tasks(batch.size());
+    std::vector<int> qlens(batch.size());
+    const size_t nseq = db->nseq;
+    for (size_t i = 0; i < batch.size(); ++i) {
+        Request& r = *batch[i];
+        Task& T = tasks[i];
+        T.prof = (const int16_t*)s->prof.p + (size_t)r.slot * s->prof_slot_elems;
+        T.qlen = (int)r.query.size();
+        T.scores = (int32_t*)s->scores.p + (size_t)r.slot * nseq;
+        T.ovf = (uint32_t*)s->ovf.p + (size_t)r.slot * s->max_slice_seqs;
+        T.pair_begin = s->pair_cut[r.next_slice];
+        T.pair_end = s->pair_cut[r.next_slice + 1];
+        T.lane_begin = s->lane_cut[r.next_slice];
+        T.lane_end = s->lane_cut[r.next_slice + 1];
+        qlens[i] = T.qlen;
+    }
+    const Shape sh = choose_shape(qlens.data(), (int)qlens.size(), s->opts.search);
+    swh_stats st{};
+    int rc = enqueue_tasks(c, db, tasks.data(), (int)tasks.size(), sh, s->opts.search,
+                           (uint32_t*)s->counters.p, &st);
+    if (rc != SWH_OK) {
+        s->loop_error = g_err;
+        return false;
+    }
+    // requests on their last slice: copy the scores out
+    std::vector<std::shared_ptr<Request>> finished;
+    for (auto& r : batch)
+        if (r->next_slice + 1 == s->opts.slices) {
+            finished.push_back(r);
+            if (hipMemcpyAsync((int32_t*)s->h_scores.p + (size_t)r->slot * nseq,
+                               (int32_t*)s->scores.p + (size_t)r->slot * nseq,
+                               nseq * sizeof(int32_t), hipMemcpyDeviceToHost,
+                               c->stream) != hipSuccess) {
+                s->loop_error = "score copy failed";
+                return false;
+            }
+        }
+    if (hipStreamSynchronize(c->stream) != hipSuccess) {
+        s->loop_error = "round failed on the device";
+        return false;
+    }
+    const double round_ms = now_ms() - t_round;
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stats.rounds += 1;
+    s->stats.busy_ms += round_ms;
+    s->stats.cells += st.cells;
+    for (size_t i = 0; i < batch.size(); ++i) {
+        Request& r = *batch[i];
+        r.next_slice += 1;
+        r.info.rounds += 1;
+        r.info.cells += (uint64_t)r.query.size() *
+                        (range_residues(db, tasks[i].pair_begin, tasks[i].pair_end) +
+                         range_residues(db, tasks[i].lane_begin, tasks[i].lane_end));
+    }
+    for (auto& r : finished) {
+        s->active.erase(std::find(s->active.begin(), s->active.end(), r));
+        s->to_complete.push_back(r);
+    }
+    if (!finished.empty()) s->cv_complete.notify_one();
+    return true;
+}
+
+void loop_main(swh_server* s) {
+    for (;;) {
+        bool ok = run_round(s);
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (!ok || (s->stop && s->active.empty())) {
+            if (!ok)
+                for (auto& r : s->active) {   // fail what is in flight
+                    r->info.status = SWH_ERR_HIP;
+                    s->to_complete.push_back(r);
+                }
+            s->active.clear();
+            for (auto& r : s->waiting) {
+                r->info.status = ok ? SWH_ERR_STATE : SWH_ERR_HIP;
+                s->to_complete.push_back(r);
+            }
+            s->waiting.clear();
+            s->stop = true;
+            s->loop_done = true;
+            s->cv_complete.notify_one();
+            return;
+        }
+    }
+}
+
+// Top-k of finished requests from their pinned score copies, then the slot
+// and its bytes go back to the pool.
+void complete_main(swh_server* s) {
+    const size_t nseq = s->db->nseq;
+    for (;;) {
+        std::shared_ptr<Request> r;
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            s->cv_complete.wait(lk, [&] { return !s->to_complete.empty() || s->loop_done; });
+            if (s->to_complete.empty()) return;   // the loop has ended and all is delivered
+            r = s->to_complete.front();
+            s->to_complete.pop_front();
+        }
+        if (r->info.status == SWH_OK && r->slot >= 0) {
+            const int32_t* sc = (const int32_t*)s->h_scores.p + (size_t)r->slot * nseq;
+            r->top_score.assign(r->k, 0);
+            r->top_index.assign(r->k, 0);
+            swh_topk(sc, nseq, r->k, r->top_score.data(), r->top_index.data());
+            if (r->want_scores) r->scores.assign(sc, sc + nseq);
+        }
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (r->slot >= 0) {
+            s->free_slots.push_back(r->slot);
+            s->bytes_in_use -= r->bytes;
+            r->slot = -1;
+        }
+        r->info.done_ms = since(s);
+        r->info.latency_ms = r->info.done_ms - r->info.submit_ms;
+        r->done = true;
+        s->stats.completed += 1;
+        s->cv_done.notify_all();
+        s->cv_work.notify_one();   // a slot came free
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int swh_server_create(swh_ctx* c, swh_db* db, const swh_server_opts* opts, swh_server** out) {
+    if (!c || !db || !out) return fail(SWH_ERR_ARG, "NULL argument");
+    if (db->ctx != c) return fail(SWH_ERR_ARG, "database belongs to another context");
+    *out = nullptr;
+    swh_server_opts o{};
+    if (opts) o = *opts;
+    if (!o.max_inflight) o.max_inflight = 64;
+    if (!o.memory_budget) o.memory_budget = uint64_t(4) << 30;
+    if (!o.slices) o.slices = 4;
+    if (!o.max_qlen) o.max_qlen = 8192;
+    if (!o.max_k) o.max_k = 100;
+    if (o.max_inflight < 1 || o.slices < 1 || o.max_qlen < 1 || o.max_k < 1)
+        return fail(SWH_ERR_ARG, "bad server options");
+    int rc = check_opts(o.search);
+    if (rc) return rc;
+    HIP_TRY(hipSetDevice(c->device));
+    auto s = std::make_unique<swh_server>();
+    s->ctx = c;
+    s->db = db;
+    s->opts = o;
+    s->t0 = now_ms();
+    s->pair_cut = cut_blocks(db, 0, db->n_pairs, o.slices);
+    s->lane_cut = cut_blocks(db, db->n_pairs, db->n_blocks, o.slices);
+    for (int i = 0; i < o.slices; ++i) {
+        const size_t n = (size_t)(s->pair_cut[i + 1] - s->pair_cut[i]) * 2 +
+                         (size_t)(s->lane_cut[i + 1] - s->lane_cut[i]) * kLaneSeqs;
+        s->max_slice_seqs = std::max(s->max_slice_seqs, n);
+    }
+    s->max_slice_seqs = std::max<size_t>(s->max_slice_seqs, 1);
+    // slots: as many as the budget holds for the longest admissible query
+    const size_t per_slot = request_bytes(s.get(), o.max_qlen);
+    const int slots = (int)std::max<uint64_t>(
+        1, std::min<uint64_t>((uint64_t)o.max_inflight, o.memory_budget / per_slot));
+    s->prof_slot_elems = (size_t)kProfRows * profile_stride(o.max_qlen);
+    const size_t nseq = std::max<size_t>(db->nseq, 1);
+    hipError_t e = s->scores.reserve((size_t)slots * nseq * sizeof(int32_t));
+    if (e == hipSuccess) e = s->ovf.reserve((size_t)slots * s->max_slice_seqs * sizeof(uint32_t));
+    if (e == hipSuccess) e = s->prof.reserve((size_t)slots * s->prof_slot_elems * sizeof(int16_t));
+    if (e == hipSuccess) e = s->counters.reserve((16 + (size_t)slots) * sizeof(uint32_t));
+    if (e == hipSuccess) e = s->h_prof.reserve((size_t)slots * s->prof_slot_elems * sizeof(int16_t));
+    if (e == hipSuccess) e = s->h_scores.reserve((size_t)slots * nseq * sizeof(int32_t));
+    if (e != hipSuccess) {
+        s->scores.release();
+        s->ovf.release();
+        s->prof.release();
+        s->counters.release();
+        s->h_prof.release();
+        s->h_scores.release();
+        return fail(SWH_ERR_NOMEM, "server buffers: %s", hipGetErrorString(e));
+    }
+    for (int i = slots - 1; i >= 0; --i) s->free_slots.push_back(i);
+    swh_server* raw = s.release();
+    raw->loop = std::thread(loop_main, raw);
+    raw->completer = std::thread(complete_main, raw);
+    *out = raw;
+    return SWH_OK;
+}
+
+int swh_server_submit(swh_server* s, const uint8_t* query, int qlen, int k, int want_scores,
+                      uint64_t* id) {
+    if (!s || !id) return fail(SWH_ERR_ARG, "NULL argument");
+    int rc = check_query(query, qlen, 0);
+    if (rc) return rc;
+    if (qlen > s->opts.max_qlen)
+        return fail(SWH_ERR_CAPACITY, "query length %d above the server's max_qlen %d", qlen,
+                    s->opts.max_qlen);
+    if (k < 0 || k > s->opts.max_k) return fail(SWH_ERR_ARG, "k = %d outside 0..%d", k, s->opts.max_k);
+    auto r = std::make_shared<Request>();
+    r->query.assign(query, query + qlen);
+    r->k = k;
+    r->want_scores = want_scores != 0;
+    r->bytes = request_bytes(s, qlen);
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->stop) return fail(SWH_ERR_STATE, "server is stopping");
+    r->id = s->next_id++;
+    r->info.submit_ms = since(s);
+    s->waiting.push_back(r);
+    s->all[r->id] = r;
+    *id = r->id;
+    s->cv_work.notify_one();
+    return SWH_OK;
+}
+
+int swh_server_wait(swh_server* s, uint64_t id, int timeout_ms, int32_t* top_score,
+                    int64_t* top_index, int32_t* scores, swh_request_info* info) {
+    if (!s) return fail(SWH_ERR_ARG, "NULL server");
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->all.find(id);
+    if (it == s->all.end()) return fail(SWH_ERR_ARG, "unknown request %llu", (unsigned long long)id);
+    std::shared_ptr<Request> r = it->second;
+    auto ready = [&] { return r->done; };
+    if (timeout_ms < 0) s->cv_done.wait(lk, ready);
+    else if (!s->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), ready))
+        return fail(SWH_ERR_STATE, "request %llu not done after %d ms", (unsigned long long)id,
+                    timeout_ms);
+    s->all.erase(id);
+    if (info) *info = r->info;
+    if (r->info.status != SWH_OK)
+        return fail(r->info.status, "request %llu failed: %s", (unsigned long long)id,
+                    s->loop_error.empty() ? "server stopped" : s->loop_error.c_str());
+    if (r->k > 0 && (!top_score || !top_index)) return fail(SWH_ERR_ARG, "top-k outputs are NULL");
+    for (int t = 0; t < r->k; ++t) {
+        top_score[t] = r->top_score[t];
+        top_index[t] = r->top_index[t];
+    }
+    if (scores && r->want_scores) std::memcpy(scores, r->scores.data(), r->scores.size() * 4);
+    return SWH_OK;
+}
+
+int swh_server_get_stats(swh_server* s, swh_server_stats* out) {
+    if (!s || !out) return fail(SWH_ERR_ARG, "NULL argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    *out = s->stats;
+    out->waiting = (int)s->waiting.size();
+    return SWH_OK;
+}
+
+int swh_server_destroy(swh_server* s) {
+    if (!s) return SWH_OK;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+        s->cv_work.notify_all();
+    }
+    if (s->loop.joinable()) s->loop.join();
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->cv_complete.notify_all();
+    }
+    if (s->completer.joinable()) s->completer.join();
+    (void)hipSetDevice(s->ctx->device);
+    s->scores.release();
+    s->ovf.release();
+    s->prof.release();
+    s->counters.release();
+    s->h_prof.release();
+    s->h_scores.release();
+    delete s;
+    return SWH_OK;
+}
+
+}  // extern "C"

@@ -123,6 +123,63 @@ class Engine:
             pass
 
 
+class Server:
+    """Continuous-batching search service on a resident database (a loop
+    thread in libswh.so).  submit() returns a request id at once; wait()
+    blocks for its top-k (and optionally its full score vector)."""
+
+    def __init__(self, engine: Engine, db: Database, max_inflight=0, memory_budget=0,
+                 slices=0, max_qlen=0, max_k=0, **search_opts):
+        o = N.SwhServerOpts(int(max_inflight), int(memory_budget), int(slices), int(max_qlen),
+                            int(max_k), _opts(**search_opts))
+        h = C.c_void_p()
+        N.check(N.lib().swh_server_create(engine.handle, db.handle, C.byref(o), C.byref(h)))
+        self.handle = h
+        self.db = db
+        self._k = {}
+        self._want = {}
+
+    def submit(self, query, k=10, want_scores=False) -> int:
+        q = _as_query(query)
+        rid = C.c_uint64()
+        N.check(N.lib().swh_server_submit(self.handle, N.ptr(q), len(q), int(k),
+                                          int(bool(want_scores)), C.byref(rid)))
+        self._k[rid.value] = int(k)
+        self._want[rid.value] = bool(want_scores)
+        return rid.value
+
+    def wait(self, rid: int, timeout_ms: int = -1):
+        k = self._k.pop(rid)
+        want = self._want.pop(rid)
+        ts = np.zeros(max(k, 1), dtype=np.int32)
+        ti = np.zeros(max(k, 1), dtype=np.int64)
+        sc = np.zeros(self.db.n, dtype=np.int32) if want else None
+        info = N.SwhRequestInfo()
+        rc = N.lib().swh_server_wait(self.handle, C.c_uint64(rid), int(timeout_ms), N.ptr(ts),
+                                     N.ptr(ti), None if sc is None else N.ptr(sc), C.byref(info))
+        if rc != 0:
+            self._k[rid] = k
+            self._want[rid] = want
+            N.check(rc)
+        return {"top_score": ts[:k], "top_index": ti[:k], "scores": sc, "info": info.as_dict()}
+
+    def stats(self):
+        st = N.SwhServerStats()
+        N.check(N.lib().swh_server_get_stats(self.handle, C.byref(st)))
+        return st.as_dict()
+
+    def close(self):
+        if self.handle:
+            N.check(N.lib().swh_server_destroy(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def topk(scores, k: int):
     """Top-k of a score vector, (score desc, index asc), padded with (0, 0)."""
     s = np.ascontiguousarray(scores, dtype=np.int32)

@@ -1,0 +1,102 @@
+"""Serving loop: continuous batching on a resident database.  A result must
+not depend on what else shares the device."""
+import threading
+import time
+
+import numpy as np
+import pytest
+
+import oracle as O
+from swhybrid_amd import datagen as G
+from swhybrid_amd.engine import Engine, Server, topk
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def setup():
+    rng = np.random.Generator(np.random.PCG64(91))
+    lens = np.concatenate([G.lognormal_lengths(rng, 3000, 300, 0.7, 20, 2000),
+                           G.loguniform_lengths(rng, 40, 2000, 6000)])
+    db = G.SeqDB.from_lengths(rng, lens)
+    eng = Engine("blosum62", 10, 2)
+    d = eng.load_db(db)
+    yield eng, db, d
+    d.close()
+    eng.close()
+
+
+def queries(n, seed):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    return [G.random_residues(rng, int(m)) for m in G.lognormal_lengths(rng, n, 300, 0.7, 30, 1500)]
+
+
+def test_alone_equals_in_the_mix(setup):
+    eng, db, d = setup
+    qs = queries(12, 92)
+    target = qs[0]
+    want = O.search_i32(target, db, eng.matrix, 10, 2)
+    srv = Server(eng, d, slices=3)
+    alone = srv.wait(srv.submit(target, k=20, want_scores=True))
+    assert np.array_equal(alone["scores"], want)
+    # now the same query among others, submitted from several threads
+    ids = {}
+    def sub(i):
+        ids[i] = srv.submit(qs[i], k=20, want_scores=(i == 0))
+    ts = [threading.Thread(target=sub, args=(i,)) for i in range(len(qs))]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join()
+    res = {i: srv.wait(ids[i]) for i in ids}
+    assert np.array_equal(res[0]["scores"], want)
+    assert np.array_equal(res[0]["top_score"], alone["top_score"])
+    assert np.array_equal(res[0]["top_index"], alone["top_index"])
+    for i, q in enumerate(qs):
+        s, ix = O.topk(O.search_i32(q, db, eng.matrix, 10, 2), 20)
+        assert np.array_equal(res[i]["top_score"], s), i
+        assert np.array_equal(res[i]["top_index"], ix), i
+    st = srv.stats()
+    assert st["completed"] == len(qs) + 1 and st["peak_inflight"] >= 2
+    srv.close()
+
+
+def test_memory_admission_serialises(setup):
+    eng, db, d = setup
+    qs = queries(5, 93)
+    # a budget that holds one request at a time: later ones wait for memory
+    srv = Server(eng, d, memory_budget=db.n * 4 + 1, slices=2)
+    ids = [srv.submit(q, k=5) for q in qs]
+    res = [srv.wait(i) for i in ids]
+    st = srv.stats()
+    assert st["peak_inflight"] == 1
+    for q, r in zip(qs, res):
+        s, ix = O.topk(O.search_i32(q, db, eng.matrix, 10, 2), 5)
+        assert np.array_equal(r["top_score"], s) and np.array_equal(r["top_index"], ix)
+        assert r["info"]["admit_ms"] >= r["info"]["submit_ms"]
+    srv.close()
+
+
+def test_slices_and_forced_rescore(setup):
+    eng, db, d = setup
+    q = queries(1, 94)[0]
+    want = O.search_i32(q, db, eng.matrix, 10, 2)
+    for slices in (1, 5):
+        srv = Server(eng, d, slices=slices, overflow_threshold=40)
+        r = srv.wait(srv.submit(q, k=10, want_scores=True))
+        assert np.array_equal(r["scores"], want)
+        assert r["info"]["rounds"] == slices
+        srv.close()
+
+
+def test_rejects_and_timeouts(setup):
+    eng, db, d = setup
+    srv = Server(eng, d, max_qlen=100, max_k=5)
+    with pytest.raises(Exception, match="max_qlen"):
+        srv.submit(np.zeros(101, dtype=np.uint8))
+    with pytest.raises(Exception):
+        srv.submit(np.zeros(10, dtype=np.uint8), k=6)
+    rid = srv.submit(np.arange(20, dtype=np.uint8) % 20, k=5)
+    r = srv.wait(rid, timeout_ms=60000)
+    assert r["info"]["status"] == 0
+    srv.close()
