@@ -68,6 +68,19 @@ def cpu_baseline(db, queries, matrix, go, ge, stride=16):
     }
 
 
+def measured_traffic():
+    """HBM bytes per k_scan launch (query length 1000) from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE runs (calibrated on a 1 GiB copy); the
+    counters cannot be read from inside a timed run."""
+    path = os.path.join(ROOT, "profiles", "k_scan_traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+        return t["per_launch_bytes"]["1000"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def run_batched(eng, d, plain_gcups):
     """64 queries, lengths log-uniform in [50, 5000], one call: one persistent
     launch whose device work queue holds every (query, block group) item."""
@@ -217,7 +230,7 @@ def main():
         "achieved": round(tops_achieved, 3), "peak": round(PEAK_INT16_TOPS, 1),
         "unit": "TOP/s (packed int16 lane-ops, 9 per cell)",
         "frac": round(tops_achieved / PEAK_INT16_TOPS, 4),
-        "traffic": None,
+        "traffic": measured_traffic(),
         "kernel_gcups": round(cells / (kernel_ms / 1e3) / 1e9, 1),
         "avg_launch_ms": round(kernel_ms / (args.steps * len(queries)), 2),
     }

@@ -311,14 +311,14 @@ int check_query(const uint8_t* q, int qlen, int i) {
 
 // Strip widths shared by a launch.  Lane scan: least padded work, each width
 // weighted by its measured cost per padded column on MI355X (5M-sequence
-// database: W=32 ~1.5% faster than 24 on long queries, W=16 ~4% slower).
-// Group scan: W=16 measured best for every query length.
+// database, pair-table profile: W=16 and W=24 within 0.5%, W=32 -- which
+// has no room for a pair table -- 6% slower).  Group scan: W=16.
 Shape choose_shape(const int* qlens, int nq, const swh_search_opts& o) {
     Shape sh{o.strip_width, o.group_width ? o.group_width : 16};
     if (!sh.Wl) {
         double best = 1e300;
-        for (int w : {24, 32, 16}) {
-            const double cost = w == 24 ? 1.0 : (w == 32 ? 0.985 : 1.04);
+        for (int w : {16, 24, 32}) {
+            const double cost = w == 32 ? 1.065 : (w == 24 ? 1.003 : 1.0);
             double cols = 0;
             for (int i = 0; i < nq; ++i) cols += (double)((qlens[i] + w - 1) / w) * w * cost;
             if (cols < best) {
@@ -404,7 +404,8 @@ int enqueue_tasks(swh_ctx* c, swh_db* db, const Task* t, int nt, const Shape& sh
     // scan: group items (long pairs) first, then lane items, one launch
     const int ps_l = std::min(std::max(max_qpad_l, 8), kSegCols) + kProfPad;
     const int ps_g = max_pass_g + kProfPad;
-    const size_t lds = (size_t)kProfRows * std::max(ps_l, ps_g) * sizeof(int16_t);
+    const size_t lds = std::max(lane_lds_bytes(sh.Wl, ps_l),
+                                (size_t)kProfRows * ps_g * sizeof(int16_t));
     const uint64_t stride_l = lane_border ? (uint64_t)db->max_rows_lane * kWave : 0;
     const uint64_t stride_g = group_border ? (uint64_t)max_gpw * db->max_rows_pair : 0;
     const uint64_t stride = std::max(stride_l, stride_g);

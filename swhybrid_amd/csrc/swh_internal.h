@@ -92,5 +92,7 @@ hipError_t launch_scan(int lane_width, int group_width, const ScanArgs& a, int g
 int scan_max_wg_per_cu(int lane_width, int group_width, size_t lds_bytes);
 hipError_t launch_rescore(const RescoreArgs& a, int grid, size_t lds_bytes, hipStream_t s);
 int rescore_max_wg_per_cu(size_t lds_bytes);
+// LDS the lane path of the search kernel needs for a strip width.
+size_t lane_lds_bytes(int lane_width, int ps_lane);
 
 }  // namespace swh

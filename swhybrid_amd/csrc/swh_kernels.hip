@@ -137,13 +137,89 @@ __device__ __forceinline__ void pair_profile(s2 (&P)[W], const short* lds, int p
     interleave_profile<W>(P, rp);
 }
 
+// Pair table: for the W columns of one strip, the interleaved profile pairs
+// (s(q_c, a), s(q_c, b)) of every residue pair (a, b), 576 rows of W dwords
+// padded to an odd number of 16-byte chunks (rows spread over LDS banks).
+// Built by the whole workgroup from the global profile once per strip; a lane
+// then reads its row's pairs directly, with no per-column interleave.
+constexpr int kPairs = kProfRows * kProfRows;
+
+template <int W>
+struct PairTab {
+    static constexpr int stride = ((W / 4) % 2 == 0) ? W + 4 : W;   // dwords per pair row
+    static constexpr size_t bytes = (size_t)kPairs * stride * sizeof(uint32_t);
+};
+
+template <int W>
+__device__ __forceinline__ void build_pair_table(uint32_t* tab, const int16_t* __restrict__ prof,
+                                                 int pstride, int c0) {
+    for (int p = threadIdx.x; p < kPairs; p += blockDim.x) {
+        const int a = p / kProfRows;
+        const int b = p - a * kProfRows;
+        const uint4* ga = reinterpret_cast<const uint4*>(prof + (size_t)a * pstride + c0);
+        const uint4* gb = reinterpret_cast<const uint4*>(prof + (size_t)b * pstride + c0);
+        uint4* out = reinterpret_cast<uint4*>(tab + (size_t)p * PairTab<W>::stride);
+#pragma unroll
+        for (int k = 0; k < W / 8; ++k) {
+            const uint4 va = ga[k];
+            const uint4 vb = gb[k];
+            out[2 * k] = make_uint4(__builtin_amdgcn_perm(vb.x, va.x, 0x05040100u),
+                                    __builtin_amdgcn_perm(vb.x, va.x, 0x07060302u),
+                                    __builtin_amdgcn_perm(vb.y, va.y, 0x05040100u),
+                                    __builtin_amdgcn_perm(vb.y, va.y, 0x07060302u));
+            out[2 * k + 1] = make_uint4(__builtin_amdgcn_perm(vb.z, va.z, 0x05040100u),
+                                        __builtin_amdgcn_perm(vb.z, va.z, 0x07060302u),
+                                        __builtin_amdgcn_perm(vb.w, va.w, 0x05040100u),
+                                        __builtin_amdgcn_perm(vb.w, va.w, 0x07060302u));
+        }
+    }
+}
+
+// Profile sources for lane_strip: the staged single profile (two residue rows
+// and a v_perm_b32 per column) or the pair table (one row, no interleave).
+template <int W>
+struct SingleSource {
+    using Raw = RawProfile<W>;
+    const short* lds;
+    int ps, jj0;
+    __device__ __forceinline__ void load(Raw& r, int a, int b) const {
+        load_profile<W>(r, lds, ps, a, b, jj0);
+    }
+    __device__ __forceinline__ void pairs(s2 (&P)[W], const Raw& r) const {
+        interleave_profile<W>(P, r);
+    }
+};
+
+template <int W>
+struct PairSource {
+    struct Raw {
+        uint4 v[W / 4];
+    };
+    const uint32_t* tab;
+    __device__ __forceinline__ void load(Raw& r, int a, int b) const {
+        const uint4* p = reinterpret_cast<const uint4*>(
+            tab + __umul24(__umul24(a, kProfRows) + b, PairTab<W>::stride));
+#pragma unroll
+        for (int k = 0; k < W / 4; ++k) r.v[k] = p[k];
+    }
+    __device__ __forceinline__ void pairs(s2 (&P)[W], const Raw& r) const {
+#pragma unroll
+        for (int k = 0; k < W / 4; ++k) {
+            P[4 * k + 0] = as_s2(r.v[k].x);
+            P[4 * k + 1] = as_s2(r.v[k].y);
+            P[4 * k + 2] = as_s2(r.v[k].z);
+            P[4 * k + 3] = as_s2(r.v[k].w);
+        }
+    }
+};
+
 // One strip of W columns over every row of a lane-mode block.  IN / OUT: the
 // strip reads / writes the border array (compile-time, so that a 4-row tile is
 // a single basic block and the compiler keeps each row's work in place).
-template <int W, bool IN, bool OUT>
+template <int W, bool IN, bool OUT, typename Src>
 __device__ __forceinline__ void lane_strip(const uint2* __restrict__ wds, int quads,
-                                           uint2* __restrict__ bord, const short* lds, int ps,
-                                           int jj0, s2 Q, s2 R, s2& S, int lane) {
+                                           uint2* __restrict__ bord, const Src& src, s2 Q, s2 R,
+                                           s2& S, int lane) {
     const s2 Z = as_s2(kZero16x2);
     s2 h[W], e[W], hq[W];
 #pragma unroll
@@ -158,8 +234,8 @@ __device__ __forceinline__ void lane_strip(const uint2* __restrict__ wds, int qu
     uint2 bn[4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) bn[r] = IN ? bord[r * kWave + lane] : zz;
-    RawProfile<W> rp;
-    load_profile<W>(rp, lds, ps, wn.x & 0xff, wn.y & 0xff, jj0);
+    typename Src::Raw rp;
+    src.load(rp, wn.x & 0xff, wn.y & 0xff);
     for (int k = 0; k < quads; ++k) {
         const uint2 w = wn;
         uint2 bc[4];
@@ -178,12 +254,12 @@ __device__ __forceinline__ void lane_strip(const uint2* __restrict__ wds, int qu
         // keep the scheduler from hoisting every load of the tile at once
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            const RawProfile<W> cur = rp;
+            const typename Src::Raw cur = rp;
             const uint2 nw = r < 3 ? w : wn;
             const int sh = 8 * ((r + 1) & 3);
-            load_profile<W>(rp, lds, ps, (nw.x >> sh) & 0xff, (nw.y >> sh) & 0xff, jj0);
+            src.load(rp, (nw.x >> sh) & 0xff, (nw.y >> sh) & 0xff);
             s2 P[W];
-            interleave_profile<W>(P, cur);
+            src.pairs(P, cur);
             const s2 hl = as_s2(bc[r].x);
             const s2 fl = as_s2(bc[r].y);
             s2 fout;
@@ -196,16 +272,16 @@ __device__ __forceinline__ void lane_strip(const uint2* __restrict__ wds, int qu
     }
 }
 
-template <int W>
+template <int W, typename Src>
 __device__ __forceinline__ void lane_strip_any(const uint2* wds, int quads, uint2* bord,
-                                               const short* lds, int ps, int jj0, bool has_in,
-                                               bool has_out, s2 Q, s2 R, s2& S, int lane) {
+                                               const Src& src, bool has_in, bool has_out, s2 Q,
+                                               s2 R, s2& S, int lane) {
     if (has_in) {
-        if (has_out) lane_strip<W, true, true>(wds, quads, bord, lds, ps, jj0, Q, R, S, lane);
-        else lane_strip<W, true, false>(wds, quads, bord, lds, ps, jj0, Q, R, S, lane);
+        if (has_out) lane_strip<W, true, true>(wds, quads, bord, src, Q, R, S, lane);
+        else lane_strip<W, true, false>(wds, quads, bord, src, Q, R, S, lane);
     } else {
-        if (has_out) lane_strip<W, false, true>(wds, quads, bord, lds, ps, jj0, Q, R, S, lane);
-        else lane_strip<W, false, false>(wds, quads, bord, lds, ps, jj0, Q, R, S, lane);
+        if (has_out) lane_strip<W, false, true>(wds, quads, bord, src, Q, R, S, lane);
+        else lane_strip<W, false, false>(wds, quads, bord, src, Q, R, S, lane);
     }
 }
 
@@ -223,6 +299,9 @@ __device__ __forceinline__ void emit(const QueryDesc& Qd, const uint32_t* perm, 
 }
 
 // One lane-mode work item: 4 consecutive lane blocks of query q, one per wave.
+// W = 16 or 24: per strip the workgroup builds the pair table in LDS; W = 32:
+// the single profile is staged per 1344-column segment (a 32-wide pair table
+// would not leave room for two workgroups per CU).
 template <int W>
 __device__ __forceinline__ void lane_item(const ScanArgs& A, const QueryDesc& Qd, int q,
                                           int local, short* lds, uint2* bord, int& loaded_q,
@@ -237,23 +316,40 @@ __device__ __forceinline__ void lane_item(const ScanArgs& A, const QueryDesc& Qd
     if (active) bd = A.blocks[blk];
     const uint2* wds = reinterpret_cast<const uint2*>(A.words + bd.word_off);
     const int quads = (int)(bd.rows >> 2);
-    const int nseg = (Qd.qpad + kSegCols - 1) / kSegCols;
     s2 S = as_s2(kZero16x2);
-    for (int seg = 0; seg < nseg; ++seg) {
-        const int seg0 = seg * kSegCols;
-        const int segw = min(kSegCols, Qd.qpad - seg0);
-        if (q != loaded_q || seg != loaded_seg) {
-            __syncthreads();
-            stage_profile(lds, Qd.prof, Qd.pstride, A.ps_lane, seg0, segw);
-            __syncthreads();
-            loaded_q = q;
-            loaded_seg = seg;
+    if constexpr (W == 32) {
+        const int nseg = (Qd.qpad + kSegCols - 1) / kSegCols;
+        for (int seg = 0; seg < nseg; ++seg) {
+            const int seg0 = seg * kSegCols;
+            const int segw = min(kSegCols, Qd.qpad - seg0);
+            if (q != loaded_q || seg != loaded_seg) {
+                __syncthreads();
+                stage_profile(lds, Qd.prof, Qd.pstride, A.ps_lane, seg0, segw);
+                __syncthreads();
+                loaded_q = q;
+                loaded_seg = seg;
+            }
+            if (active) {
+                for (int c0 = seg0; c0 < seg0 + segw; c0 += W) {
+                    const SingleSource<W> src{lds, A.ps_lane, c0 - seg0};
+                    lane_strip_any<W>(wds, quads, bord, src, c0 > 0, c0 + W < Qd.qpad, Q, R, S,
+                                      lane);
+                }
+            }
         }
-        if (active) {
-            for (int c0 = seg0; c0 < seg0 + segw; c0 += W)
-                lane_strip_any<W>(wds, quads, bord, lds, A.ps_lane, c0 - seg0, c0 > 0,
-                                  c0 + W < Qd.qpad, Q, R, S, lane);
+    } else {
+        uint32_t* tab = reinterpret_cast<uint32_t*>(lds);
+        for (int c0 = 0; c0 < Qd.qpad; c0 += W) {
+            __syncthreads();
+            build_pair_table<W>(tab, Qd.prof, Qd.pstride, c0);
+            __syncthreads();
+            if (active) {
+                const PairSource<W> src{tab};
+                lane_strip_any<W>(wds, quads, bord, src, c0 > 0, c0 + W < Qd.qpad, Q, R, S,
+                                  lane);
+            }
         }
+        loaded_q = -1;   // the single-profile staging area was overwritten
     }
     if (active) {
         emit(Qd, A.perm, bd.first + lane, (int)S.x + 32768);
@@ -610,5 +706,13 @@ int scan_max_wg_per_cu(int wl, int wg, size_t lds) {
 }
 
 int rescore_max_wg_per_cu(size_t lds) { return occ(k_rescore, lds); }
+
+size_t lane_lds_bytes(int width, int ps_lane) {
+    switch (width) {
+        case 16: return PairTab<16>::bytes;
+        case 24: return PairTab<24>::bytes;
+        default: return (size_t)kProfRows * ps_lane * sizeof(int16_t);
+    }
+}
 
 }  // namespace swh
