@@ -53,6 +53,13 @@ typedef struct swh_stats {
     uint64_t n_launches;      /* kernel launches issued by the call */
 } swh_stats;
 
+/* Where the lane scan reads the query profile from (swh_search_opts.profile_mode). */
+#define SWH_PROFILE_AUTO 0
+#define SWH_PROFILE_SINGLE 1   /* per-residue profile rows staged in LDS; a row of the
+                                  database needs two of them and a byte permute */
+#define SWH_PROFILE_PAIR 2     /* residue-pair table in LDS, built per strip
+                                  (strip widths 16 and 24 only) */
+
 /* Per-call tuning.  Zero-initialise for defaults. */
 typedef struct swh_search_opts {
     int strip_width;          /* query columns held in registers per lane:
@@ -63,7 +70,11 @@ typedef struct swh_search_opts {
                                  (1..64); 0 = auto */
     int group_width;          /* query columns per lane in group mode: 0 = auto,
                                  else 8 or 16 */
-    int use_graph;            /* batched calls: capture launches in a hipGraph */
+    int use_graph;            /* capture the launch sequence in a hipGraph and
+                                 replay it while the query lengths, database and
+                                 options stay the same */
+    int profile_mode;         /* SWH_PROFILE_*; auto = tuning table, else pair
+                                 table where the width allows */
 } swh_search_opts;
 
 /* Database layout options.  Zero-initialise for defaults. */
@@ -116,6 +127,50 @@ int swh_search_batch(swh_ctx* ctx, swh_db* db, const uint8_t* const* queries,
  * deterministic for ties. */
 int swh_topk(const int32_t* scores, uint64_t n, int k, int32_t* out_score,
              int64_t* out_index);
+
+/* ---- Autotuning -------------------------------------------------------------
+ * The launch shape is chosen per query-length bucket from measurements on a
+ * length-stratified sample of the database: lanes per sequence (the database
+ * length from which sequences are searched by a group of lanes, and the group
+ * size per query), the query-profile placement and the register tile depth
+ * (query columns per lane).  No reference counterpart: the reference's shapes
+ * are compile-time constants (src/CUDAKernel.cu:152-192, src/AVXKernel.cpp). */
+#define SWH_TUNE_MAX 16
+/* Lane-scan configurations measured per bucket, in lane_cost order:
+ * (16, pair) (24, pair) (16, single) (24, single) (32, single). */
+#define SWH_TUNE_LANE_CONFIGS 5
+
+/* One bucket: queries of length <= qlen_max (rows ascend; the last row also
+ * takes longer queries). */
+typedef struct swh_tune_entry {
+    int qlen_max;
+    int strip_width;          /* best lane-scan tile depth (16, 24, 32) */
+    int profile_mode;         /* best profile placement (SWH_PROFILE_SINGLE / _PAIR) */
+    int group_lanes;          /* best lanes per long sequence pair (1..64) */
+    int group_width;          /* best group-mode tile depth (8, 16) */
+    float lane_cost[SWH_TUNE_LANE_CONFIGS];   /* measured launch ms per padded
+                                                 query column, per configuration */
+} swh_tune_entry;
+
+typedef struct swh_tune_result {
+    uint32_t group_min_len;   /* best swh_db_opts.group_min_len among those tried */
+    int n_entries;
+    swh_tune_entry entries[SWH_TUNE_MAX];
+    double seconds;           /* wall time of the tuning */
+    uint64_t sample_seqs, sample_residues;
+} swh_tune_result;
+
+/* Measure on every sample_every-th sequence in length order (0 = 8), with
+ * queries_per_bucket random queries of each bucket length (0 = 4) and reps
+ * timed repetitions (0 = 1), then install the table on ctx.  bucket_qlens
+ * ascending, 1..SWH_TUNE_MAX of them. */
+int swh_autotune(swh_ctx* ctx, const uint8_t* residues, const uint64_t* offsets,
+                 const uint32_t* lengths, uint64_t nseq, int sample_every,
+                 const int* bucket_qlens, int n_buckets, int queries_per_bucket, int reps,
+                 swh_tune_result* out);
+/* Install a table (n = 0 clears it: built-in shape choice). */
+int swh_set_tuning(swh_ctx* ctx, const swh_tune_entry* entries, int n);
+int swh_get_tuning(const swh_ctx* ctx, swh_tune_entry* entries, int cap, int* n);
 
 /* ---- Serving: continuous batching on a resident database ------------------
  * A server owns a loop thread.  Requests are admitted, in arrival order, when

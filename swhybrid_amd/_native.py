@@ -22,8 +22,14 @@ EXPORTS = (
     "swh_ctx_destroy", "swh_ctx_set_scoring", "swh_db_create", "swh_db_destroy",
     "swh_db_info", "swh_search", "swh_search_batch", "swh_topk",
     "swh_server_create", "swh_server_submit", "swh_server_wait", "swh_server_get_stats",
-    "swh_server_destroy",
+    "swh_server_destroy", "swh_autotune", "swh_set_tuning", "swh_get_tuning",
 )
+
+SWH_PROFILE_AUTO, SWH_PROFILE_SINGLE, SWH_PROFILE_PAIR = 0, 1, 2
+SWH_TUNE_MAX = 16
+SWH_TUNE_LANE_CONFIGS = 5
+# (strip width, pair table) of the lane_cost columns, as in include/swh.h
+LANE_CONFIGS = ((16, True), (24, True), (16, False), (24, False), (32, False))
 
 
 class SwhStats(C.Structure):
@@ -49,6 +55,7 @@ class SwhSearchOpts(C.Structure):
         ("group_lanes", C.c_int),
         ("group_width", C.c_int),
         ("use_graph", C.c_int),
+        ("profile_mode", C.c_int),
     ]
 
 
@@ -86,6 +93,27 @@ class SwhServerStats(C.Structure):
 
 class SwhDbOpts(C.Structure):
     _fields_ = [("group_min_len", C.c_uint32)]
+
+
+class SwhTuneEntry(C.Structure):
+    _fields_ = [
+        ("qlen_max", C.c_int), ("strip_width", C.c_int), ("profile_mode", C.c_int),
+        ("group_lanes", C.c_int), ("group_width", C.c_int),
+        ("lane_cost", C.c_float * SWH_TUNE_LANE_CONFIGS),
+    ]
+
+    def as_dict(self):
+        d = {f: getattr(self, f) for f, _ in self._fields_ if f != "lane_cost"}
+        d["lane_cost"] = [float(v) for v in self.lane_cost]
+        return d
+
+
+class SwhTuneResult(C.Structure):
+    _fields_ = [
+        ("group_min_len", C.c_uint32), ("n_entries", C.c_int),
+        ("entries", SwhTuneEntry * SWH_TUNE_MAX), ("seconds", C.c_double),
+        ("sample_seqs", C.c_uint64), ("sample_residues", C.c_uint64),
+    ]
 
 
 class SwhError(RuntimeError):
@@ -126,6 +154,10 @@ def lib():
     L.swh_server_wait.argtypes = [vp, C.c_uint64, C.c_int, vp, vp, vp, C.POINTER(SwhRequestInfo)]
     L.swh_server_get_stats.argtypes = [vp, C.POINTER(SwhServerStats)]
     L.swh_server_destroy.argtypes = [vp]
+    L.swh_autotune.argtypes = [vp, vp, vp, vp, C.c_uint64, C.c_int, vp, C.c_int, C.c_int,
+                               C.c_int, C.POINTER(SwhTuneResult)]
+    L.swh_set_tuning.argtypes = [vp, C.POINTER(SwhTuneEntry), C.c_int]
+    L.swh_get_tuning.argtypes = [vp, C.POINTER(SwhTuneEntry), C.c_int, C.POINTER(C.c_int)]
     for name in EXPORTS:
         if name != "swh_last_error" and name != "swh_version":
             getattr(L, name).restype = C.c_int

@@ -3,6 +3,7 @@
 // the public ABI (include/swh.h).
 #pragma once
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cstdarg>
 #include <cstdint>
@@ -125,6 +126,22 @@ using swhh::DevBuf;
 using swhh::PinnedBuf;
 using swh::BlockDesc;
 
+// A captured launch sequence and what it was captured for.
+struct GraphCache {
+    hipGraphExec_t exec = nullptr;
+    std::vector<long long> key;
+    swh_stats st{};
+    void* desc = nullptr;
+    void* border = nullptr;
+    void* h_desc = nullptr;
+    std::vector<char> desc_bytes;   // the pinned descriptor tables the graph uploads
+    void release() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        exec = nullptr;
+        key.clear();
+    }
+};
+
 struct swh_ctx {
     int device = 0;
     int num_cus = 0;
@@ -133,12 +150,17 @@ struct swh_ctx {
     int gap_open = 10, gap_extend = 2, max_score = 0;
     DevBuf border, counters, prof, scores, ovf, desc;
     PinnedBuf h_prof, h_scores, h_desc, h_ctr;
+    size_t desc_used = 0;             // bytes of h_desc the last enqueue uploaded
     hipEvent_t ev[8] = {};
     size_t border_budget = size_t(8) << 30;   // bytes of HBM for strip borders
+    GraphCache graph;
+    std::vector<swh_tune_entry> tune;   // empty: built-in shape choice
+    uint64_t tune_gen = 0;              // bumped whenever the table changes
 };
 
 struct swh_db {
     swh_ctx* ctx = nullptr;
+    uint64_t uid = 0;           // unique per database ever created in the process
     std::vector<uint64_t> blk_real;     // prefix sums over blocks of real residues
     std::vector<uint64_t> blk_padded;   // ... and of padded residues (rows x sequences)
     uint64_t nseq = 0, residues = 0, padded = 0;
@@ -166,7 +188,16 @@ struct Task {
 
 struct Shape {
     int Wl, Wg;
+    bool pair;   // lane scan reads the residue-pair table
 };
+
+// Lane-scan configurations of a tuning table's lane_cost columns.
+struct LaneCfg {
+    int width;
+    bool pair;
+};
+constexpr LaneCfg kLaneCfgs[SWH_TUNE_LANE_CONFIGS] = {
+    {16, true}, {24, true}, {16, false}, {24, false}, {32, false}};
 
 
 int profile_stride(int qlen);
@@ -175,8 +206,11 @@ uint64_t range_residues(const swh_db* db, int b0, int b1);
 uint64_t range_padded(const swh_db* db, int b0, int b1);
 int check_opts(const swh_search_opts& o);
 int check_query(const uint8_t* q, int qlen, int i);
-Shape choose_shape(const int* qlens, int nq, const swh_search_opts& o);
+Shape choose_shape(const swh_ctx* c, const int* qlens, int nq, const swh_search_opts& o);
 int enqueue_tasks(swh_ctx* c, swh_db* db, const Task* t, int nt, const Shape& sh,
                   const swh_search_opts& o, uint32_t* counters, swh_stats* st);
+// Search every query over the whole database; scores_host gets nq rows of nseq.
+int run_queries(swh_ctx* c, swh_db* db, const uint8_t* const* qs, const int* qlens, int nq,
+                const swh_search_opts& o, int32_t* scores_host, swh_stats* stats);
 
 }  // namespace swhh

@@ -28,7 +28,7 @@ struct BlockDesc {
 
 // One query of a launch, as one kernel sees it.  Kernels take a table of
 // these, so one persistent launch serves many queries through one device work
-// queue; the work items of query q are [item_begin, item_begin + n_items).
+// queue; query q owns n_items items of its table's kind.
 struct QueryDesc {
     const int16_t* prof;     // profile [24][pstride] int16, zero beyond qlen
     int32_t* scores;         // caller order, one per database sequence
@@ -38,7 +38,6 @@ struct QueryDesc {
     int qpad;                // qlen rounded up to this kernel's strip width
     int pstride;             // profile row stride (int16), >= qpad
     int group;               // group kernel: lanes per group (1..64)
-    int item_begin;
     int n_items;
     int ovf_threshold;       // score >= threshold -> int32 rescore
     int blk_begin, blk_end;  // blocks of this query (pair blocks in the group
@@ -51,23 +50,25 @@ struct ScanArgs {
     const uint32_t* perm;      // sorted position -> caller index (kNoSeq for fillers)
     const QueryDesc* group_q;  // per query: group-item range, G, qpad for the group width
     const QueryDesc* lane_q;   // per query: lane-item range, qpad for the lane width
+    const uint2* items;        // work queue order: x = query | kLaneItem, y = item of the query
     uint32_t* work_counter;
     uint2* border;             // per wave slot, border_stride uint2
     uint64_t border_stride;
     uint32_t border_rows;      // group items: rows reserved per group (longest pair)
     int n_queries;
-    int n_items_group, n_items_lane;
+    int n_items;
     int ps_lane, ps_group;     // LDS profile row strides (int16)
     int gap_oe, gap_e;
 };
 
-// Work items of the search kernel: group items first (G lanes cooperate on
-// one long sequence pair; lane g of a group holds query columns
-// [pass*G*W + g*W, +W) and works on row t-g at step t, its last column's H/F
-// reaching lane g+1 one step later through a wave-wide DPP shift; a wave
-// holds floor(64/G) groups, an item is 4 waves of pairs), then lane items (4
-// consecutive lane blocks of one query, one per wave).  Rescore items are 256
-// overflow-list entries.
+// Work items of the search kernel: group items (G lanes cooperate on one long
+// sequence pair; lane g of a group holds query columns [pass*G*W + g*W, +W)
+// and works on row t-g at step t, its last column's H/F reaching lane g+1 one
+// step later through a wave-wide DPP shift; a wave holds floor(64/G) groups,
+// an item is 4 waves of pairs) and lane items (4 consecutive lane blocks of
+// one query, one per wave), handed out longest first across every query of
+// the launch.  Rescore items are 256 overflow-list entries.
+constexpr uint32_t kLaneItem = 0x80000000u;
 
 struct RescoreArgs {
     const uint32_t* words;
@@ -87,12 +88,13 @@ struct RescoreArgs {
 
 // Launchers (swh_kernels.hip).
 hipError_t launch_reset(uint32_t* p, int n, hipStream_t s);
-hipError_t launch_scan(int lane_width, int group_width, const ScanArgs& a, int grid,
+// pair: the lane scan reads a per-strip residue-pair table (widths 16, 24).
+hipError_t launch_scan(int lane_width, int group_width, bool pair, const ScanArgs& a, int grid,
                        size_t lds_bytes, hipStream_t s);
-int scan_max_wg_per_cu(int lane_width, int group_width, size_t lds_bytes);
+int scan_max_wg_per_cu(int lane_width, int group_width, bool pair, size_t lds_bytes);
 hipError_t launch_rescore(const RescoreArgs& a, int grid, size_t lds_bytes, hipStream_t s);
 int rescore_max_wg_per_cu(size_t lds_bytes);
 // LDS the lane path of the search kernel needs for a strip width.
-size_t lane_lds_bytes(int lane_width, int ps_lane);
+size_t lane_lds_bytes(int lane_width, bool pair, int ps_lane);
 
 }  // namespace swh

@@ -299,10 +299,11 @@ __device__ __forceinline__ void emit(const QueryDesc& Qd, const uint32_t* perm, 
 }
 
 // One lane-mode work item: 4 consecutive lane blocks of query q, one per wave.
-// W = 16 or 24: per strip the workgroup builds the pair table in LDS; W = 32:
-// the single profile is staged per 1344-column segment (a 32-wide pair table
-// would not leave room for two workgroups per CU).
-template <int W>
+// PAIR (W = 16 or 24): per strip the workgroup builds the pair table in LDS;
+// otherwise the single profile is staged per 1344-column segment and each row
+// interleaves two residue rows (a 32-wide pair table would not leave room for
+// two workgroups per CU, so W = 32 always stages the single profile).
+template <int W, bool PAIR>
 __device__ __forceinline__ void lane_item(const ScanArgs& A, const QueryDesc& Qd, int q,
                                           int local, short* lds, uint2* bord, int& loaded_q,
                                           int& loaded_seg) {
@@ -317,7 +318,7 @@ __device__ __forceinline__ void lane_item(const ScanArgs& A, const QueryDesc& Qd
     const uint2* wds = reinterpret_cast<const uint2*>(A.words + bd.word_off);
     const int quads = (int)(bd.rows >> 2);
     s2 S = as_s2(kZero16x2);
-    if constexpr (W == 32) {
+    if constexpr (!PAIR) {
         const int nseg = (Qd.qpad + kSegCols - 1) / kSegCols;
         for (int seg = 0; seg < nseg; ++seg) {
             const int seg0 = seg * kSegCols;
@@ -472,35 +473,20 @@ __device__ __forceinline__ void group_item(const ScanArgs& A, const QueryDesc& Q
     }
 }
 
-// The search kernel: one persistent launch, one device work queue.  Items
-// [0, n_items_group) are group items (long pairs, longest first), then
-// [n_items_group, +n_items_lane) lane items; each range is query-major and
-// the query of an item is found in the group or lane query table.  Long work
-// therefore starts first and short blocks fill in around it.
+// The search kernel: one persistent launch, one device work queue.  The host
+// orders the items longest first across every query of the launch (group
+// items of long pairs and lane items alike), so long work starts first and
+// short blocks fill in around it.
 __device__ __forceinline__ bool next_scan_item(const ScanArgs& A, int* s_buf, int& local,
                                                int& q, bool& is_group) {
     __syncthreads();
     if (threadIdx.x == 0) {
         const int it = (int)atomicAdd(A.work_counter, 1u);
-        int kind = 0, rel = it;
-        const QueryDesc* qs = A.group_q;
-        if (it >= A.n_items_group) {
-            kind = 1;
-            rel = it - A.n_items_group;
-            qs = A.lane_q;
-        }
-        int lo = 0, hi = A.n_queries - 1;
-        const bool live = it < A.n_items_group + A.n_items_lane;
-        if (live) {
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if (qs[mid].item_begin <= rel) lo = mid;
-                else hi = mid - 1;
-            }
-        }
-        s_buf[0] = live ? rel - qs[lo].item_begin : -1;
-        s_buf[1] = lo;
-        s_buf[2] = kind;
+        uint2 v = make_uint2(0u, 0xffffffffu);
+        if (it < A.n_items) v = A.items[it];
+        s_buf[0] = (int)v.y;
+        s_buf[1] = (int)(v.x & ~kLaneItem);
+        s_buf[2] = (v.x & kLaneItem) ? 1 : 0;
     }
     __syncthreads();
     local = s_buf[0];
@@ -509,8 +495,9 @@ __device__ __forceinline__ bool next_scan_item(const ScanArgs& A, int* s_buf, in
     return local >= 0;
 }
 
-template <int WL, int WG>
+template <int WL, int WG, bool PAIR>
 __global__ __launch_bounds__(256) void k_scan(ScanArgs A) {
+    static_assert(!PAIR || WL == 16 || WL == 24, "pair table only for W = 16, 24");
     extern __shared__ __align__(16) short lds[];
     __shared__ int s_buf[3];
     __shared__ uint32_t s_red[kWavesPerWG][kWave];
@@ -526,7 +513,7 @@ __global__ __launch_bounds__(256) void k_scan(ScanArgs A) {
             loaded_q = -1;   // the group item restaged the profile
         } else {
             const QueryDesc Qd = A.lane_q[q];
-            lane_item<WL>(A, Qd, q, local, lds, slot, loaded_q, loaded_seg);
+            lane_item<WL, PAIR>(A, Qd, q, local, lds, slot, loaded_q, loaded_seg);
         }
     }
 }
@@ -669,17 +656,20 @@ hipError_t launch_reset(uint32_t* p, int n, hipStream_t s) {
     return hipGetLastError();
 }
 
-template <int WL, int WG>
+template <int WL, int WG, bool PAIR>
 static hipError_t launch_scan_w(const ScanArgs& a, int grid, size_t lds, hipStream_t s) {
-    hipLaunchKernelGGL((k_scan<WL, WG>), dim3(grid), dim3(256), lds, s, a);
+    hipLaunchKernelGGL((k_scan<WL, WG, PAIR>), dim3(grid), dim3(256), lds, s, a);
     return hipGetLastError();
 }
 
-#define SWH_SCAN_CASES(X) X(16, 8) X(16, 16) X(24, 8) X(24, 16) X(32, 8) X(32, 16)
+#define SWH_SCAN_CASES(X)                                                                  \
+    X(16, 8, true) X(16, 16, true) X(24, 8, true) X(24, 16, true) X(16, 8, false)          \
+    X(16, 16, false) X(24, 8, false) X(24, 16, false) X(32, 8, false) X(32, 16, false)
 
-hipError_t launch_scan(int wl, int wg, const ScanArgs& a, int grid, size_t lds, hipStream_t s) {
-#define X(L, G) \
-    if (wl == L && wg == G) return launch_scan_w<L, G>(a, grid, lds, s);
+hipError_t launch_scan(int wl, int wg, bool pair, const ScanArgs& a, int grid, size_t lds,
+                       hipStream_t s) {
+#define X(L, G, P) \
+    if (wl == L && wg == G && pair == P) return launch_scan_w<L, G, P>(a, grid, lds, s);
     SWH_SCAN_CASES(X)
 #undef X
     return hipErrorInvalidValue;
@@ -697,9 +687,9 @@ static int occ(K kern, size_t lds) {
     return n > 0 ? n : 1;
 }
 
-int scan_max_wg_per_cu(int wl, int wg, size_t lds) {
-#define X(L, G) \
-    if (wl == L && wg == G) return occ(k_scan<L, G>, lds);
+int scan_max_wg_per_cu(int wl, int wg, bool pair, size_t lds) {
+#define X(L, G, P) \
+    if (wl == L && wg == G && pair == P) return occ(k_scan<L, G, P>, lds);
     SWH_SCAN_CASES(X)
 #undef X
     return 1;
@@ -707,12 +697,10 @@ int scan_max_wg_per_cu(int wl, int wg, size_t lds) {
 
 int rescore_max_wg_per_cu(size_t lds) { return occ(k_rescore, lds); }
 
-size_t lane_lds_bytes(int width, int ps_lane) {
-    switch (width) {
-        case 16: return PairTab<16>::bytes;
-        case 24: return PairTab<24>::bytes;
-        default: return (size_t)kProfRows * ps_lane * sizeof(int16_t);
-    }
+size_t lane_lds_bytes(int width, bool pair, int ps_lane) {
+    if (pair && width == 16) return PairTab<16>::bytes;
+    if (pair && width == 24) return PairTab<24>::bytes;
+    return (size_t)kProfRows * ps_lane * sizeof(int16_t);
 }
 
 }  // namespace swh

@@ -145,7 +145,7 @@ bool run_round(swh_server* s) {
         T.lane_end = s->lane_cut[r.next_slice + 1];
         qlens[i] = T.qlen;
     }
-    const Shape sh = choose_shape(qlens.data(), (int)qlens.size(), s->opts.search);
+    const Shape sh = choose_shape(c, qlens.data(), (int)qlens.size(), s->opts.search);
     swh_stats st{};
     int rc = enqueue_tasks(c, db, tasks.data(), (int)tasks.size(), sh, s->opts.search,
                            (uint32_t*)s->counters.p, &st);

@@ -34,9 +34,15 @@ def _as_query(q) -> np.ndarray:
     return q
 
 
-def _opts(strip_width=0, overflow_threshold=0, group_lanes=0, group_width=0, use_graph=0):
+_PROFILE_MODES = {0: 0, "auto": 0, 1: 1, "single": 1, 2: 2, "pair": 2}
+
+
+def _opts(strip_width=0, overflow_threshold=0, group_lanes=0, group_width=0, use_graph=0,
+          profile_mode=0):
+    if profile_mode not in _PROFILE_MODES:
+        raise ValueError(f"profile_mode {profile_mode!r}: use 'auto', 'single' or 'pair'")
     return N.SwhSearchOpts(int(strip_width), int(overflow_threshold), int(group_lanes),
-                           int(group_width), int(use_graph))
+                           int(group_width), int(use_graph), _PROFILE_MODES[profile_mode])
 
 
 class Database:
@@ -110,6 +116,46 @@ class Engine:
         N.check(N.lib().swh_search_batch(self.handle, db.handle, arr, N.ptr(lens), len(qs),
                                          C.byref(o), N.ptr(out), C.byref(st)))
         return out, st.as_dict()
+
+    def autotune(self, db: SeqDB, bucket_qlens, sample_every=8, queries_per_bucket=4, reps=1):
+        """Measure launch shapes per query-length bucket on every
+        sample_every-th sequence of db in length order, install the table on
+        this engine and return it with the best group_min_len for load_db."""
+        res = np.ascontiguousarray(db.residues, dtype=np.uint8)
+        off = np.ascontiguousarray(db.offsets, dtype=np.uint64)
+        lens = np.ascontiguousarray(db.lengths, dtype=np.uint32)
+        b = np.ascontiguousarray(np.asarray(bucket_qlens, dtype=np.int32))
+        out = N.SwhTuneResult()
+        N.check(N.lib().swh_autotune(self.handle, N.ptr(res), N.ptr(off), N.ptr(lens), db.n,
+                                     int(sample_every), N.ptr(b), len(b),
+                                     int(queries_per_bucket), int(reps), C.byref(out)))
+        return {
+            "group_min_len": int(out.group_min_len),
+            "entries": [out.entries[i].as_dict() for i in range(out.n_entries)],
+            "seconds": float(out.seconds),
+            "sample_seqs": int(out.sample_seqs),
+            "sample_residues": int(out.sample_residues),
+        }
+
+    def set_tuning(self, entries):
+        """Install a tuning table (list of dicts as autotune returns; [] clears)."""
+        arr = (N.SwhTuneEntry * max(1, len(entries)))()
+        for i, e in enumerate(entries):
+            t = arr[i]
+            for f in ("qlen_max", "strip_width", "profile_mode", "group_lanes", "group_width"):
+                setattr(t, f, int(e[f]))
+            cost = list(e["lane_cost"])
+            if len(cost) != N.SWH_TUNE_LANE_CONFIGS:
+                raise ValueError("lane_cost needs one value per lane configuration")
+            for k, v in enumerate(cost):
+                t.lane_cost[k] = float(v)
+        N.check(N.lib().swh_set_tuning(self.handle, arr, len(entries)))
+
+    def get_tuning(self):
+        arr = (N.SwhTuneEntry * N.SWH_TUNE_MAX)()
+        n = C.c_int(0)
+        N.check(N.lib().swh_get_tuning(self.handle, arr, N.SWH_TUNE_MAX, C.byref(n)))
+        return [arr[i].as_dict() for i in range(n.value)]
 
     def close(self):
         if self.handle:

@@ -37,13 +37,32 @@ def check(eng, db, q, **opts):
     return got, st
 
 
-@pytest.mark.parametrize("width", [16, 24, 32])
-def test_c1_slice_all_widths(eng, width):
+@pytest.mark.parametrize("width,placement", [(16, "pair"), (24, "pair"), (32, "single"),
+                                             (16, "single"), (24, "single")])
+def test_c1_slice_all_widths(eng, width, placement):
     db = small_db(1500, 11)
     q = G.c1_query()
-    _, st = check(eng, db, q, strip_width=width)
+    _, st = check(eng, db, q, strip_width=width, profile_mode=placement)
     assert st["n_overflow"] == 0
     assert st["cells"] == db.total * len(q)
+
+
+@pytest.mark.parametrize("qlen", [5, 1345, 2700])
+def test_single_profile_long_and_short_queries(eng, qlen):
+    """Single-profile placement at W=16/24 crosses the 1344-column staging
+    segments for long queries."""
+    db = small_db(300, 500 + qlen, mean=250, hi=900)
+    q = G.queries([qlen], 600 + qlen)[0]
+    for w in (16, 24):
+        check(eng, db, q, strip_width=w, profile_mode="single")
+
+
+def test_pair_table_rejected_at_width_32(eng):
+    db = small_db(10, 3)
+    d = eng.load_db(db)
+    with pytest.raises(Exception, match="pair"):
+        eng.search(d, G.queries([20], 1)[0], strip_width=32, profile_mode="pair")
+    d.close()
 
 
 @pytest.mark.parametrize("qlen", [1, 2, 7, 33, 100, 1343, 1344, 1345, 2700])
@@ -208,3 +227,111 @@ def test_batch_empty_and_one(eng):
     q = G.queries([64], 84)[0]
     got, _ = eng.search_batch(d, [q])
     assert np.array_equal(got[0], O.search_i32(q, db, eng.matrix, 10, 2))
+
+
+# --- captured launch sequences --------------------------------------------------
+
+def test_graph_replays_match_eager(eng):
+    """use_graph: the first call of a shape runs eagerly and captures; later calls
+    with the same lengths replay.  Eager searches of other shapes run between
+    replays (reusing the staging buffers); every replay is checked."""
+    rng = np.random.Generator(np.random.PCG64(91))
+    db = long_db(92, n=120, lo=20, hi=3000)
+    d = eng.load_db(db, group_min_len=1500)
+    lens = (40, 500, 1300)
+    for rep in range(4):
+        qs = [G.random_residues(rng, m) for m in lens]
+        if rep == 2:
+            qs[1] = G.mutate(rng, db.seq(5))[:500]          # a strong hit
+            qs[1] = np.concatenate([qs[1], G.random_residues(rng, 500 - len(qs[1]))])
+        got, st = eng.search_batch(d, qs, use_graph=1)
+        for i, q in enumerate(qs):
+            want = O.search_i32(q, db, eng.matrix, 10, 2)
+            assert np.array_equal(got[i], want), f"replay {rep} query {i}"
+        assert st["cells"] == sum(db.total * len(q) for q in qs)
+        # other work between replays: a different shape and a single search
+        other = [G.random_residues(rng, 77), G.random_residues(rng, 2100)]
+        g2, _ = eng.search_batch(d, other)
+        assert np.array_equal(g2[0], O.search_i32(other[0], db, eng.matrix, 10, 2))
+        s1, _ = eng.search(d, other[1], strip_width=32)
+        assert np.array_equal(s1, g2[1])
+    # forced rescore through a captured graph
+    qs = [G.random_residues(rng, m) for m in lens]
+    for rep in range(2):
+        got, st = eng.search_batch(d, qs, use_graph=1, overflow_threshold=30)
+        want = [O.search_i32(q, db, eng.matrix, 10, 2) for q in qs]
+        n_ovf = sum(int((w >= 30).sum()) for w in want)
+        assert n_ovf > 0 and st["n_overflow"] == n_ovf
+        for i in range(len(qs)):
+            assert np.array_equal(got[i], want[i]), f"forced rescore, replay {rep} query {i}"
+    d.close()
+
+
+# --- autotuner ---------------------------------------------------------------------
+
+def test_autotune_installs_a_table_and_scores_stay_exact():
+    e = Engine("blosum62", 10, 2)
+    rng = np.random.Generator(np.random.PCG64(101))
+    lens = np.concatenate([G.lognormal_lengths(rng, 3000, 300, 0.7, 20, 2000),
+                           G.loguniform_lengths(rng, 40, 2000, 6000)])
+    db = G.SeqDB.from_lengths(rng, lens)
+    res = e.autotune(db, [64, 400, 1500], sample_every=2, queries_per_bucket=2)
+    assert res["sample_seqs"] == (db.n + 1) // 2
+    assert res["group_min_len"] in (1024, 2048, 4096, 8192, 0xFFFFFFFF)
+    ents = res["entries"]
+    assert [t["qlen_max"] for t in ents] == [64, 400, 1500]
+    for t in ents:
+        assert t["strip_width"] in (16, 24, 32) and t["profile_mode"] in (1, 2)
+        assert t["group_width"] in (8, 16) and 0 <= t["group_lanes"] <= 64
+        assert all(c > 0 for c in t["lane_cost"])
+    assert e.get_tuning() == ents
+    d = e.load_db(db, group_min_len=res["group_min_len"] if res["group_min_len"] < 2**31 else -1)
+    qs = [G.random_residues(rng, m) for m in (30, 300, 1200, 3000)]
+    got, _ = e.search_batch(d, qs)
+    for i, q in enumerate(qs):
+        assert np.array_equal(got[i], O.search_i32(q, db, e.matrix, 10, 2)), f"query {i}"
+    d.close()
+    e.close()
+
+
+@pytest.mark.parametrize("cfg", range(5))
+def test_tuning_table_drives_the_shape(cfg):
+    """A table whose only measured lane configuration is cfg forces it; group
+    lanes and widths come from the buckets.  Scores equal the oracle's."""
+    from swhybrid_amd import _native as N
+    e = Engine("blosum62", 10, 2)
+    w, pair = N.LANE_CONFIGS[cfg]
+    cost = [0.0] * 5
+    cost[cfg] = 1.0
+    table = [dict(qlen_max=200, strip_width=w, profile_mode=2 if pair else 1, group_lanes=3,
+                  group_width=8, lane_cost=cost),
+             dict(qlen_max=2000, strip_width=w, profile_mode=2 if pair else 1, group_lanes=0,
+                  group_width=16, lane_cost=cost)]
+    e.set_tuning(table)
+    assert e.get_tuning() == table
+    db = long_db(700 + cfg, n=80, lo=20, hi=3000)
+    d = e.load_db(db, group_min_len=1000)
+    rng = np.random.Generator(np.random.PCG64(701))
+    qs = [G.random_residues(rng, m) for m in (50, 150, 900, 2500)]
+    got, _ = e.search_batch(d, qs)
+    for i, q in enumerate(qs):
+        assert np.array_equal(got[i], O.search_i32(q, db, e.matrix, 10, 2)), f"query {i}"
+    e.set_tuning([])
+    assert e.get_tuning() == []
+    d.close()
+    e.close()
+
+
+def test_tuning_table_validation():
+    e = Engine("blosum62", 10, 2)
+    good = dict(qlen_max=100, strip_width=16, profile_mode=2, group_lanes=0, group_width=16,
+                lane_cost=[1, 1, 1, 1, 1])
+    for bad in (dict(strip_width=20), dict(profile_mode=0), dict(group_width=12),
+                dict(strip_width=32, profile_mode=2), dict(lane_cost=[0, 0, 0, 0, 0]),
+                dict(group_lanes=65)):
+        with pytest.raises(Exception):
+            e.set_tuning([{**good, **bad}])
+    with pytest.raises(Exception):
+        e.set_tuning([good, {**good, "qlen_max": 50}])     # not ascending
+    e.set_tuning([good])
+    e.close()
