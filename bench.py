@@ -69,14 +69,15 @@ def cpu_baseline(db, queries, matrix, go, ge, stride=16):
 
 
 def measured_traffic():
-    """HBM bytes per k_scan launch (query length 1000) from the committed
-    rocprofv3 FETCH_SIZE / WRITE_SIZE runs (calibrated on a 1 GiB copy); the
-    counters cannot be read from inside a timed run."""
+    """HBM bytes per k_scan launch, averaged over the five query lengths of
+    this workload, from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs
+    (calibrated on a 1 GiB copy); the counters cannot be read from inside a
+    timed run."""
     path = os.path.join(ROOT, "profiles", "k_scan_traffic.json")
     try:
         with open(path) as f:
             t = json.load(f)
-        return t["per_launch_bytes"]["1000"]
+        return t["mean_per_launch_bytes"]
     except (OSError, KeyError, ValueError):
         return None
 
@@ -129,6 +130,38 @@ def run_batched(eng, db, d, plain_gcups, group_min_len):
                          "buckets": [{k: (round(v, 6) if isinstance(v, float) else v)
                                       for k, v in t.items() if k != "lane_cost"}
                                      for t in tune["entries"]]}}, d
+
+
+def run_small_db(eng):
+    """Where batching pays: the 100k-sequence database (29.9M residues) is too
+    small for one query to fill 256 CUs (780 lane blocks, 195 work items per
+    query against 768 workgroup slots), so the 64 queries of the batched
+    workload searched one call each leave most of the GPU idle, while one
+    batched launch keeps every CU busy."""
+    from swhybrid_amd import datagen as G
+    db = G.c1_db()
+    d = eng.load_db(db)
+    qs = G.c3_queries()
+    for q in qs[:4]:
+        eng.search(d, q)
+    t0 = time.perf_counter()
+    cells = 0
+    for q in qs:
+        _, st = eng.search(d, q)
+        cells += st["cells"]
+    plain = time.perf_counter() - t0
+    eng.search_batch(d, qs, use_graph=1)                # eager run, then capture
+    t0 = time.perf_counter()
+    _, st = eng.search_batch(d, qs, use_graph=1)        # graph replay
+    batched = time.perf_counter() - t0
+    d.close()
+    log(f"small database: plain {cells / plain / 1e9:.0f} GCUPS, batched "
+        f"{st['cells'] / batched / 1e9:.0f} GCUPS")
+    return {"workload": "100k sequences (29.9M residues, the CPU-runnable database) x the 64 "
+                        "batched queries: one call per query vs one batched call",
+            "plain_gcups": round(cells / plain / 1e9, 1),
+            "batched_gcups": round(st["cells"] / batched / 1e9, 1),
+            "batched_vs_plain": round(plain / batched, 2)}
 
 
 def run_serving(eng, d, qps_levels, n_requests):
@@ -286,6 +319,7 @@ def main():
     }
     if not args.no_batched:
         out["batched"], d = run_batched(eng, db, d, gcups, args.group_min_len)
+        out["batched_small_db"] = run_small_db(eng)
     if not args.no_serve:
         out["served"] = run_serving(eng, d, [float(x) for x in args.serve_qps.split(",")],
                                     args.serve_requests)

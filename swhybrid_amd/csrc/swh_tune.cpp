@@ -136,7 +136,7 @@ int swh_autotune(swh_ctx* c, const uint8_t* residues, const uint64_t* offsets,
         qptr[i] = qstore[i].data();
         qlen[i] = (int)qstore[i].size();
     }
-    std::vector<int32_t> scores((size_t)std::max(nb, qpb) * ns);
+    std::vector<int32_t> scores((size_t)nb * qpb * ns);
 
     // tune with the built-in shape choice, restore the caller's table on failure
     std::vector<swh_tune_entry> saved = c->tune;
@@ -168,19 +168,16 @@ int swh_autotune(swh_ctx* c, const uint8_t* residues, const uint64_t* offsets,
     };
 
     // 1. lanes per sequence, database side: which lengths go to group mode.
-    // One query of each bucket per launch; thresholds that give the same
-    // layout as a shorter one are skipped.
-    std::vector<const uint8_t*> one_ptr(nb);
-    std::vector<int> one_len(nb);
-    for (int b = 0; b < nb; ++b) {
-        one_ptr[b] = qptr[(size_t)b * qpb];
-        one_len[b] = qlen[(size_t)b * qpb];
-    }
+    // Every query of every bucket in one launch, so that the launch is long
+    // next to its longest item (a lane block of the longest lane-mode
+    // sequences) and the timing reflects throughput, as in a batched call;
+    // thresholds that give the same layout as a shorter one are skipped.
     const uint32_t kThresholds[] = {1024u, 2048u, 4096u, 8192u, 0xffffffffu};
     uint32_t best_min = 0xffffffffu;
     double best_ms = 1e300;
     uint64_t prev_long = ~0ull;
     bool warmed = false;
+    const int nall = (int)qptr.size();
     for (uint32_t thr : kThresholds) {
         uint64_t n_long = 0;
         for (uint32_t L : smp.lengths) n_long += L >= thr;
@@ -190,18 +187,14 @@ int swh_autotune(swh_ctx* c, const uint8_t* residues, const uint64_t* offsets,
         int rc = make_db(thr, h);
         if (rc) return restore(rc);
         swh_search_opts o{};
-        swh_stats st{};
-        if (!warmed) {   // first launch of the process pays code-object loading
-            rc = run_queries(c, h.db, one_ptr.data(), one_len.data(), nb, o, scores.data(), &st);
+        if (!warmed) {   // the first launch of the process pays code-object loading
+            swh_stats st{};
+            rc = run_queries(c, h.db, qptr.data(), qlen.data(), qpb, o, scores.data(), &st);
             if (rc) return restore(rc);
             warmed = true;
         }
-        double ms = 1e300;
-        for (int r = 0; r < reps; ++r) {
-            rc = run_queries(c, h.db, one_ptr.data(), one_len.data(), nb, o, scores.data(), &st);
-            if (rc) return restore(rc);
-            ms = std::min(ms, st.kernel_ms);
-        }
+        double ms;
+        if ((rc = timed(h.db, 0, nall, o, &ms))) return restore(rc);
         if (ms < best_ms) {
             best_ms = ms;
             best_min = thr;
@@ -211,6 +204,7 @@ int swh_autotune(swh_ctx* c, const uint8_t* residues, const uint64_t* offsets,
 
     // 2. per bucket, on the chosen layout: lane-scan width and profile
     // placement, then group size and group width with that lane shape
+    // (qpb queries of the bucket's length per launch)
     DbHolder h;
     int rc = make_db(best_min, h);
     if (rc) return restore(rc);
