@@ -1,8 +1,9 @@
 /*
- * swhybrid_host.h -- C entry point of the host worker pipeline
- * (libswhybrid_host.so): an in-memory DataPool feeds HIPWorker threads (the
- * Worker plugin that reaches libswh.so), a RecalcWorker collects anything a
- * worker could not score, and the per-worker top-k lists are merged.
+ * swhybrid_host.h -- C entry points of the host side (libswhybrid_host.so):
+ * a DataPool (in memory, or over the reference's database files) feeds
+ * HIPWorker threads (the Worker plugin that reaches libswh.so), a
+ * RecalcWorker collects anything a worker could not score, and the
+ * per-worker top-k lists are merged; plus the DBmaker and FASTA input.
  *
  * Replaces the reference's process entry (src/main.cpp:7-16: Param, DataPool,
  * WorkerPool constructed from argv, results printed by ~WorkerPool,
@@ -34,6 +35,53 @@ int swhy_pool_search(const uint8_t* residues, const uint64_t* offsets, const uin
                      int n_workers, int32_t* out_top_score, int64_t* out_top_index,
                      int64_t* out_top_pool_index, int32_t* out_scores, double* out_seconds,
                      double* out_gcups);
+
+/* ---- The reference's on-disk database -------------------------------------
+ * <base>.seq / .map / .title as the reference's DBmaker writes them
+ * (src/FastaFile.cpp:72-359) and its DataPool reads them
+ * (src/DataPool.cpp:145-303).  A sequence's index is its position in .seq,
+ * the all-pad fillers of the final batch included. */
+typedef struct swhy_dbfile swhy_dbfile;
+
+/* The DBmaker (src/DBmaker.cpp, src/FastaFile.cpp, src/Bucket.cpp): FASTA ->
+ * <out_base>.seq/.map/.title in the same format, with the mixed-length final
+ * batches padded per sequence (the reference pads them at shifted positions,
+ * src/Bucket.cpp:123-130).  flush_bytes 0 = 256 MB. */
+int swhy_dbfile_make(const char* fasta_path, const char* out_base, uint64_t flush_bytes,
+                     uint64_t* out_records, uint64_t* out_slots);
+/* Maps <base>.seq read-only and reads the map; replaces DataPool::run /
+ * initRead (src/DataPool.cpp:79-160). */
+int swhy_dbfile_open(const char* base, swhy_dbfile** out);
+int swhy_dbfile_info(const swhy_dbfile* f, uint64_t* records, uint64_t* slots,
+                     uint64_t* residues, uint64_t* batches);
+/* Per index: residues (codes, valid while f is open) and length without the
+ * trailing pad codes (0 for fillers): ready for swh_db_create. */
+int swhy_dbfile_arrays(const swhy_dbfile* f, const uint8_t** residues, const uint64_t** offsets,
+                       const uint32_t** lengths);
+/* The '>' line of sequence index, NUL-terminated and cut to cap bytes; *len
+ * gets its full length; "" for fillers.  Replaces DataPool::getTitle
+ * (src/DataPool.cpp:289-310), without its dangling pointer. */
+int swhy_dbfile_title(swhy_dbfile* f, uint64_t index, char* buf, uint64_t cap, uint64_t* len);
+int swhy_dbfile_close(swhy_dbfile* f);
+/* swhy_pool_search over the file's batches in file order, as the reference's
+ * DataPool::fillChunk hands them out (src/DataPool.cpp:163-286); indices are
+ * file indices. */
+int swhy_dbfile_pool_search(swhy_dbfile* f, const uint8_t* query, int qlen,
+                            const int32_t* matrix, int gap_open, int gap_extend, int topk,
+                            uint64_t chunk_bytes, int n_workers, int32_t* out_top_score,
+                            int64_t* out_top_index, int32_t* out_scores, double* out_seconds,
+                            double* out_gcups);
+
+/* ---- FASTA queries ------------------------------------------------------------
+ * Every record of a FASTA file (the reference reads the first only,
+ * src/DataPool.cpp:312-344), letters translated with its table: 0..22,
+ * '@' = 23, other letters 0xFF (which the search rejects in a query). */
+typedef struct swhy_fasta swhy_fasta;
+int swhy_fasta_open(const char* path, swhy_fasta** out);
+int swhy_fasta_count(const swhy_fasta* f, uint64_t* n);
+int swhy_fasta_record(const swhy_fasta* f, uint64_t i, const uint8_t** codes, uint64_t* len,
+                      const char** title);
+int swhy_fasta_close(swhy_fasta* f);
 
 #ifdef __cplusplus
 }

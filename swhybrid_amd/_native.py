@@ -165,7 +165,10 @@ def lib():
     return L
 
 
-HOST_EXPORTS = ("swhy_last_error", "swhy_pool_search")
+HOST_EXPORTS = ("swhy_last_error", "swhy_pool_search", "swhy_dbfile_make", "swhy_dbfile_open",
+                "swhy_dbfile_info", "swhy_dbfile_arrays", "swhy_dbfile_title",
+                "swhy_dbfile_close", "swhy_dbfile_pool_search", "swhy_fasta_open",
+                "swhy_fasta_count", "swhy_fasta_record", "swhy_fasta_close")
 _host = None
 
 
@@ -185,6 +188,23 @@ def host_lib():
     L.swhy_pool_search.argtypes = [vp, vp, vp, C.c_uint64, vp, C.c_int, vp, C.c_int, C.c_int,
                                    C.c_int, C.c_uint64, C.c_int, vp, vp, vp, vp,
                                    C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    u64p = C.POINTER(C.c_uint64)
+    L.swhy_dbfile_make.argtypes = [C.c_char_p, C.c_char_p, C.c_uint64, u64p, u64p]
+    L.swhy_dbfile_open.argtypes = [C.c_char_p, C.POINTER(vp)]
+    L.swhy_dbfile_info.argtypes = [vp, u64p, u64p, u64p, u64p]
+    L.swhy_dbfile_arrays.argtypes = [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp)]
+    L.swhy_dbfile_title.argtypes = [vp, C.c_uint64, C.c_char_p, C.c_uint64, u64p]
+    L.swhy_dbfile_close.argtypes = [vp]
+    L.swhy_dbfile_pool_search.argtypes = [vp, vp, C.c_int, vp, C.c_int, C.c_int, C.c_int,
+                                          C.c_uint64, C.c_int, vp, vp, vp,
+                                          C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.swhy_fasta_open.argtypes = [C.c_char_p, C.POINTER(vp)]
+    L.swhy_fasta_count.argtypes = [vp, u64p]
+    L.swhy_fasta_record.argtypes = [vp, C.c_uint64, C.POINTER(vp), u64p,
+                                    C.POINTER(C.c_char_p)]
+    L.swhy_fasta_close.argtypes = [vp]
+    for name in HOST_EXPORTS[1:]:
+        getattr(L, name).restype = C.c_int
     _host = L
     return L
 

@@ -27,6 +27,20 @@ DataPool::DataPool(const uint8_t* residues, const uint64_t* offsets, const uint3
     }
 }
 
+DataPool::DataPool(const swhy::DbFile& db, const uint8_t* query, size_t qlen)
+    : residues_(db.residue_data()), offsets_(db.offsets()), lengths_(db.lengths()),
+      query_(query, query + qlen) {
+    if (db.slots() >= UINT32_MAX) throw "DataPool: too many sequences\n";
+    order_.resize(db.slots());
+    std::iota(order_.begin(), order_.end(), 0u);
+    for (const auto& b : db.batches()) {
+        if (b.num == 0) continue;
+        batches_.push_back(Batch{b.first, b.num, b.len});
+        totalPadded_ += (size_t)b.num * b.len;
+        maxBatchBytes_ = std::max(maxBatchBytes_, (size_t)b.num * b.len);
+    }
+}
+
 size_t DataPool::maxInfoPerChunk(size_t bufferSize) const {
     // every batch holds at least 4 bytes per sequence and one sequence
     return std::min(batches_.size(), bufferSize / 4 + 1);

@@ -3,22 +3,26 @@
 // Interface and chunk format of the reference's DataPool (src/DataPool.h:12-75,
 // fillChunk src/DataPool.cpp:280-286): a chunk is a run of consecutive batches,
 // each batch numSeqs sequences of one padded length stored back to back, and
-// baseIndex is the pool index of the chunk's first sequence.  The pool is
-// built in memory from the caller's sequences (sorted longest first and
-// grouped into batches of up to 128 sequences of equal length rounded up to a
-// multiple of 4, padded with code 23) instead of being read from .seq files.
+// baseIndex is the pool index of the chunk's first sequence.  Two sources:
+// the caller's sequences in memory (sorted longest first and grouped into
+// batches of up to 128 sequences of equal length rounded up to a multiple of
+// 4, padded with code 23), or a database in the reference's .seq/.map/.title
+// files, whose batches are handed out in file order as the reference does.
 #pragma once
 #include <cstdint>
 #include <mutex>
 #include <string>
 #include <vector>
 
+#include "DbFile.h"
 #include "Defs.h"
 
 class DataPool {
   public:
     DataPool(const uint8_t* residues, const uint64_t* offsets, const uint32_t* lengths,
              uint64_t nseq, const uint8_t* query, size_t qlen);
+    // Pool index = file index (fillers included).
+    DataPool(const swhy::DbFile& db, const uint8_t* query, size_t qlen);
 
     // Copies the next batches into dst while they fit in bufferSize bytes.
     // Returns false (infoSize = 0) when the pool is exhausted.
@@ -33,7 +37,8 @@ class DataPool {
     size_t maxInfoPerChunk(size_t bufferSize) const;
 
     // Caller's index of pool sequence poolIdx (the reference looks titles up
-    // by pool index, src/DataPool.cpp:289-303; here the caller holds titles).
+    // by pool index, src/DataPool.cpp:289-303): the file index for a file
+    // database.
     uint64_t originalIndex(size_t poolIdx) const { return order_[poolIdx]; }
     void rewind();
 

@@ -1,11 +1,17 @@
-// host_capi.cpp -- C entry point that runs the whole worker pipeline
-// (DataPool -> HIPWorker(s) -> RecalcWorker -> merged ResultList), the way the
-// reference's main() drives WorkerPool (src/main.cpp:7-16), for callers that
-// hold the database in memory.
+// host_capi.cpp -- C entry points of the host side: the whole worker
+// pipeline (DataPool -> HIPWorker(s) -> RecalcWorker -> merged ResultList),
+// the way the reference's main() drives WorkerPool (src/main.cpp:7-16), over
+// a database in memory or in the reference's .seq/.map/.title files; the
+// DBmaker; and FASTA query input.
+#include <algorithm>
+#include <cstring>
+#include <exception>
+#include <memory>
 #include <string>
 #include <vector>
 
 #include "DataPool.h"
+#include "DbFile.h"
 #include "Param.h"
 #include "WorkerPool.h"
 #include "swhybrid_host.h"
@@ -13,6 +19,55 @@
 namespace {
 thread_local std::string g_err;
 }
+
+namespace {
+
+// Runs the worker pipeline over pool; top-k and scores by pool index
+// translated through pool.originalIndex.
+int run_pool(DataPool& pool, uint64_t nseq, const int32_t* matrix, int gap_open, int gap_extend,
+             int topk, uint64_t chunk_bytes, int n_workers, int32_t* out_top_score,
+             int64_t* out_top_index, int64_t* out_top_pool_index, int32_t* out_scores,
+             double* out_seconds, double* out_gcups) {
+    std::vector<int> m(matrix, matrix + 1024);
+    Param params(m.data(), gap_open, gap_extend, topk, n_workers);
+    std::vector<int> sink;
+    if (out_scores) sink.assign(nseq, -1);
+    double secs = 0, gcups = 0;
+    std::vector<Result<int>> top;
+    {
+        WorkerPool wp(&pool, &params, chunk_bytes ? chunk_bytes : (size_t(256) << 20));
+        const int rc = wp.run(out_scores ? sink.data() : nullptr);
+        if (rc != 0) {
+            g_err = "worker pipeline: " + wp.error();
+            return -2;
+        }
+        secs = wp.seconds();
+        gcups = wp.gcups();
+        top = wp.result().getResult();
+    }
+    for (int t = 0; t < topk; ++t) {
+        const bool real = t < (int)top.size() && top[t].score > 0;
+        out_top_score[t] = t < (int)top.size() ? top[t].score : 0;
+        out_top_index[t] = real ? (int64_t)pool.originalIndex(top[t].idx) : 0;
+        if (out_top_pool_index) out_top_pool_index[t] = real ? (int64_t)top[t].idx : 0;
+    }
+    if (out_scores)
+        for (uint64_t p = 0; p < nseq; ++p) out_scores[pool.originalIndex(p)] = sink[p];
+    if (out_seconds) *out_seconds = secs;
+    if (out_gcups) *out_gcups = gcups;
+    return 0;
+}
+
+}  // namespace
+
+struct swhy_dbfile {
+    std::unique_ptr<swhy::DbFile> f;
+};
+
+struct swhy_fasta {
+    std::vector<std::string> titles;
+    std::vector<std::vector<uint8_t>> codes;
+};
 
 extern "C" {
 
@@ -30,35 +85,170 @@ int swhy_pool_search(const uint8_t* residues, const uint64_t* offsets, const uin
         g_err = "swhy_pool_search: bad arguments";
         return -1;
     }
-    DataPool pool(residues, offsets, lengths, nseq, query, (size_t)qlen);
-    std::vector<int> m(matrix, matrix + 1024);
-    Param params(m.data(), gap_open, gap_extend, topk, n_workers);
-    std::vector<int> sink;
-    if (out_scores) sink.assign(nseq, -1);
-    int rc;
-    double secs = 0, gcups = 0;
-    std::vector<Result<int>> top;
-    {
-        WorkerPool wp(&pool, &params, chunk_bytes ? chunk_bytes : (size_t(256) << 20));
-        rc = wp.run(out_scores ? sink.data() : nullptr);
-        if (rc != 0) {
-            g_err = "swhy_pool_search: " + wp.error();
-            return -2;
+    try {
+        DataPool pool(residues, offsets, lengths, nseq, query, (size_t)qlen);
+        return run_pool(pool, nseq, matrix, gap_open, gap_extend, topk, chunk_bytes, n_workers,
+                        out_top_score, out_top_index, out_top_pool_index, out_scores,
+                        out_seconds, out_gcups);
+    } catch (const char* e) {
+        g_err = std::string("swhy_pool_search: ") + e;
+    } catch (const std::exception& e) {
+        g_err = std::string("swhy_pool_search: ") + e.what();
+    }
+    return -2;
+}
+
+int swhy_dbfile_make(const char* fasta_path, const char* out_base, uint64_t flush_bytes,
+                     uint64_t* out_records, uint64_t* out_slots) {
+    if (!fasta_path || !out_base) {
+        g_err = "swhy_dbfile_make: NULL path";
+        return -1;
+    }
+    try {
+        const swhy::DbMakeStats st =
+            swhy::make_db_files(fasta_path, out_base, flush_bytes ? flush_bytes : (uint64_t(1) << 28));
+        if (out_records) *out_records = st.records;
+        if (out_slots) *out_slots = st.slots;
+        return 0;
+    } catch (const std::exception& e) {
+        g_err = std::string("swhy_dbfile_make: ") + e.what();
+        return -2;
+    }
+}
+
+int swhy_dbfile_open(const char* base, swhy_dbfile** out) {
+    if (!base || !out) {
+        g_err = "swhy_dbfile_open: NULL argument";
+        return -1;
+    }
+    *out = nullptr;
+    try {
+        auto* h = new swhy_dbfile();
+        h->f.reset(new swhy::DbFile(base));
+        *out = h;
+        return 0;
+    } catch (const std::exception& e) {
+        g_err = std::string("swhy_dbfile_open: ") + e.what();
+        return -2;
+    }
+}
+
+int swhy_dbfile_info(const swhy_dbfile* h, uint64_t* records, uint64_t* slots,
+                     uint64_t* residues, uint64_t* batches) {
+    if (!h) {
+        g_err = "swhy_dbfile_info: NULL handle";
+        return -1;
+    }
+    if (records) *records = h->f->records();
+    if (slots) *slots = h->f->slots();
+    if (residues) *residues = h->f->residues();
+    if (batches) *batches = h->f->batches().size();
+    return 0;
+}
+
+int swhy_dbfile_arrays(const swhy_dbfile* h, const uint8_t** residues, const uint64_t** offsets,
+                       const uint32_t** lengths) {
+    if (!h || !residues || !offsets || !lengths) {
+        g_err = "swhy_dbfile_arrays: NULL argument";
+        return -1;
+    }
+    *residues = h->f->residue_data();
+    *offsets = h->f->offsets();
+    *lengths = h->f->lengths();
+    return 0;
+}
+
+int swhy_dbfile_title(swhy_dbfile* h, uint64_t index, char* buf, uint64_t cap, uint64_t* len) {
+    if (!h || (cap > 0 && !buf)) {
+        g_err = "swhy_dbfile_title: NULL argument";
+        return -1;
+    }
+    const std::string t = h->f->title(index);
+    if (len) *len = t.size();
+    if (cap > 0) {
+        const uint64_t n = std::min<uint64_t>(cap - 1, t.size());
+        std::memcpy(buf, t.data(), n);
+        buf[n] = 0;
+    }
+    return 0;
+}
+
+int swhy_dbfile_close(swhy_dbfile* h) {
+    delete h;
+    return 0;
+}
+
+int swhy_dbfile_pool_search(swhy_dbfile* h, const uint8_t* query, int qlen,
+                            const int32_t* matrix, int gap_open, int gap_extend, int topk,
+                            uint64_t chunk_bytes, int n_workers, int32_t* out_top_score,
+                            int64_t* out_top_index, int32_t* out_scores, double* out_seconds,
+                            double* out_gcups) {
+    if (!h || !query || qlen <= 0 || !matrix || topk < 0 || n_workers < 1 ||
+        (topk > 0 && (!out_top_score || !out_top_index))) {
+        g_err = "swhy_dbfile_pool_search: bad arguments";
+        return -1;
+    }
+    try {
+        DataPool pool(*h->f, query, (size_t)qlen);
+        return run_pool(pool, h->f->slots(), matrix, gap_open, gap_extend, topk, chunk_bytes,
+                        n_workers, out_top_score, out_top_index, nullptr, out_scores,
+                        out_seconds, out_gcups);
+    } catch (const char* e) {
+        g_err = std::string("swhy_dbfile_pool_search: ") + e;
+    } catch (const std::exception& e) {
+        g_err = std::string("swhy_dbfile_pool_search: ") + e.what();
+    }
+    return -2;
+}
+
+int swhy_fasta_open(const char* path, swhy_fasta** out) {
+    if (!path || !out) {
+        g_err = "swhy_fasta_open: NULL argument";
+        return -1;
+    }
+    *out = nullptr;
+    try {
+        swhy::FastaReader r(path);
+        auto* h = new swhy_fasta();
+        std::string title, letters;
+        while (r.next(title, letters)) {
+            std::vector<uint8_t> c(letters.size());
+            for (size_t i = 0; i < letters.size(); ++i)
+                c[i] = swhy::residue_code((unsigned char)letters[i]);
+            h->titles.push_back(title);
+            h->codes.push_back(std::move(c));
         }
-        secs = wp.seconds();
-        gcups = wp.gcups();
-        top = wp.result().getResult();
+        *out = h;
+        return 0;
+    } catch (const std::exception& e) {
+        g_err = std::string("swhy_fasta_open: ") + e.what();
+        return -2;
     }
-    for (int t = 0; t < topk; ++t) {
-        const bool real = t < (int)top.size() && top[t].score > 0;
-        out_top_score[t] = t < (int)top.size() ? top[t].score : 0;
-        out_top_index[t] = real ? (int64_t)pool.originalIndex(top[t].idx) : 0;
-        if (out_top_pool_index) out_top_pool_index[t] = real ? (int64_t)top[t].idx : 0;
+}
+
+int swhy_fasta_count(const swhy_fasta* h, uint64_t* n) {
+    if (!h || !n) {
+        g_err = "swhy_fasta_count: NULL argument";
+        return -1;
     }
-    if (out_scores)
-        for (uint64_t p = 0; p < nseq; ++p) out_scores[pool.originalIndex(p)] = sink[p];
-    if (out_seconds) *out_seconds = secs;
-    if (out_gcups) *out_gcups = gcups;
+    *n = h->titles.size();
+    return 0;
+}
+
+int swhy_fasta_record(const swhy_fasta* h, uint64_t i, const uint8_t** codes, uint64_t* len,
+                      const char** title) {
+    if (!h || i >= h->titles.size()) {
+        g_err = "swhy_fasta_record: no such record";
+        return -1;
+    }
+    if (codes) *codes = h->codes[i].data();
+    if (len) *len = h->codes[i].size();
+    if (title) *title = h->titles[i].c_str();
+    return 0;
+}
+
+int swhy_fasta_close(swhy_fasta* h) {
+    delete h;
     return 0;
 }
 
