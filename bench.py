@@ -164,6 +164,57 @@ def run_small_db(eng):
             "batched_vs_plain": round(plain / batched, 2)}
 
 
+def run_dbfile(eng):
+    """The reference's database files: the 100k-sequence database written as
+    FASTA, made into .seq/.map/.title, mapped, and searched through the worker
+    pipeline (DataPool over the file's batches -> HIPWorker), the reference's
+    own path for one query, with the top-10 checked against the resident
+    search of the same sequences."""
+    import tempfile
+    from swhybrid_amd import alphabet as A
+    from swhybrid_amd import datagen as G
+    from swhybrid_amd import dbfile as F
+    from swhybrid_amd.engine import topk
+    db = G.c1_db()
+    q = G.c1_query()
+    letters = np.frombuffer(A.CODE_ORDER.encode() + b"@", dtype=np.uint8)
+    with tempfile.TemporaryDirectory() as tmp:
+        fa = os.path.join(tmp, "c1.fa")
+        with open(fa, "wb") as f:
+            text = letters[db.residues]
+            for i in range(db.n):
+                o = int(db.offsets[i])
+                f.write(b">seq%d\n" % i + text[o:o + int(db.lengths[i])].tobytes() + b"\n")
+        fa_bytes = os.path.getsize(fa)
+        t0 = time.perf_counter()
+        records, slots = F.make(fa, os.path.join(tmp, "c1"))
+        make_s = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        f = F.DbFile(os.path.join(tmp, "c1"))
+        fdb = f.seqdb()
+        open_s = time.perf_counter() - t0
+        f.pool_search(q, eng.matrix, k=10)                       # warm
+        r = f.pool_search(q, eng.matrix, k=10)
+        d = eng.load_db(fdb)
+        scores, _ = eng.search(d, q)
+        d.close()
+        ts, ti = topk(scores, 10)
+        same = bool((ts == r["top_score"]).all() and (ti == r["top_index"]).all())
+        # each hit's title names the generated sequence whose residues it holds
+        titles_ok = all(np.array_equal(fdb.seq(int(i)), db.seq(int(f.title(int(i))[4:])))
+                        for i in ti)
+        f.close()
+    log(f"database files: make {fa_bytes / make_s / 1e6:.0f} MB/s, open {open_s * 1e3:.0f} ms, "
+        f"pipeline {r['gcups']:.0f} GCUPS")
+    return {"workload": "100k-sequence database (29.9M residues) as FASTA -> .seq/.map/.title -> "
+                        "worker pipeline over the file's batches, query length 256, top-10",
+            "fasta_mb": round(fa_bytes / 1e6, 1), "make_s": round(make_s, 2),
+            "make_mb_per_s": round(fa_bytes / make_s / 1e6, 1), "open_ms": round(open_s * 1e3, 1),
+            "records": records, "slots": slots,
+            "pipeline_gcups": round(r["gcups"], 1), "pipeline_s": round(r["seconds"], 3),
+            "top10_equals_resident_search": same, "titles_ok": titles_ok}
+
+
 def run_serving(eng, d, qps_levels, n_requests):
     """Poisson arrivals at fixed QPS, lengths logN(mean 400, sigma 0.7) clipped
     to [30, 5000], top-20 per request, the database resident; p50/p99 latency
@@ -227,6 +278,7 @@ def main():
                     help="comma-separated subset of the workload's query lengths (profiling runs)")
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-query batched run")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving run")
+    ap.add_argument("--no-dbfile", action="store_true", help="skip the database-file run")
     ap.add_argument("--serve-qps", type=str, default="5,20,50")
     ap.add_argument("--serve-requests", type=int, default=60)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -320,6 +372,8 @@ def main():
     if not args.no_batched:
         out["batched"], d = run_batched(eng, db, d, gcups, args.group_min_len)
         out["batched_small_db"] = run_small_db(eng)
+    if not args.no_dbfile:
+        out["dbfile"] = run_dbfile(eng)
     if not args.no_serve:
         out["served"] = run_serving(eng, d, [float(x) for x in args.serve_qps.split(",")],
                                     args.serve_requests)

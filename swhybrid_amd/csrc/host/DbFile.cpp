@@ -56,9 +56,13 @@ struct Bucket {
     // Moves the first k sequences out, each padded at its end to out_len.
     void take(uint64_t k, uint32_t out_len, std::string& bytes, std::string& text) {
         if (k == 0) return;
-        for (uint64_t i = 0; i < k; ++i) {
-            bytes.append(seq, i * len, len);
-            bytes.append(out_len - len, (char)kPad);
+        if (out_len == len) {
+            bytes.append(seq, 0, k * len);
+        } else {
+            for (uint64_t i = 0; i < k; ++i) {
+                bytes.append(seq, i * len, len);
+                bytes.append(out_len - len, (char)kPad);
+            }
         }
         const uint64_t cut = ends[k - 1];
         text.append(titles, 0, cut);
@@ -205,8 +209,12 @@ bool FastaReader::next(std::string& title, std::string& residues) {
             have_pending_ = true;
             break;
         }
-        for (char c : line)
-            if (c != ' ' && c != '\t') residues.push_back(c);
+        if (line.find_first_of(" \t") == std::string::npos) {
+            residues += line;
+        } else {
+            for (char c : line)
+                if (c != ' ' && c != '\t') residues.push_back(c);
+        }
     }
     return true;
 }
@@ -223,9 +231,10 @@ DbMakeStats make_db_files(const std::string& fasta_path, const std::string& base
         // pad to a multiple of 4 with '@'; an empty record becomes 4 pads
         size_t padded = (letters.size() + 3) & ~size_t(3);
         if (padded == 0) padded = 4;
-        codes.assign(padded, (char)kPad);
+        codes.resize(padded);
         for (size_t i = 0; i < letters.size(); ++i)
-            codes[i] = (char)residue_code((unsigned char)letters[i]);
+            codes[i] = (char)kCodes.t[(unsigned char)letters[i]];
+        for (size_t i = letters.size(); i < padded; ++i) codes[i] = (char)kPad;
         if (padded > UINT32_MAX) throw std::runtime_error("sequence too long");
         auto it = by_len.find((uint32_t)padded);
         if (it == by_len.end()) {
