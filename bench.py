@@ -215,6 +215,33 @@ def run_dbfile(eng):
             "top10_equals_resident_search": same, "titles_ok": titles_ok}
 
 
+def run_hybrid(db, qlens=(1000, 5478)):
+    """The reference's hybrid scheduling: the worker pipeline over the same
+    5M-sequence database held in host memory, chunks of 256 MB to the
+    HIPWorker and of 16 MB to the CPU worker (the host's AVX2 cores), all
+    from one DataPool; HIPWorker alone first, then both."""
+    from swhybrid_amd import datagen as G
+    from swhybrid_amd.engine import hybrid_search
+    threads = min(16, os.cpu_count() or 1)
+    out = {"workload": "worker pipeline over the 5M-sequence database in host memory: "
+                       "HIPWorker alone vs HIPWorker + CPU worker on one DataPool",
+           "cpu_threads": threads}
+    for L in qlens:
+        q = G.queries([L], 5000 + L)[0]
+        a = hybrid_search(db, q, k=TOP_K, hip_workers=1, cpu_threads=0)
+        b = hybrid_search(db, q, k=TOP_K, hip_workers=1, cpu_threads=threads)
+        same = bool((a["top_score"] == b["top_score"]).all())
+        ra, rb = a["report"], b["report"]
+        out[str(L)] = {"hip_only_gcups": round(ra["gcups"], 1), "hybrid_gcups": round(rb["gcups"], 1),
+                       "cpu_share": round(rb["cpu_residues"] / max(1, rb["cpu_residues"] + rb["hip_residues"]), 4),
+                       "cpu_worker_gcups": round(rb["cpu_residues"] * L / rb["cpu_busy_s"] / 1e9, 1)
+                       if rb["cpu_busy_s"] > 0 else 0.0,
+                       "top20_scores_equal": same}
+        log(f"hybrid qlen {L}: HIP alone {ra['gcups']:.0f}, HIP + CPU {rb['gcups']:.0f} GCUPS "
+            f"(CPU share {out[str(L)]['cpu_share']:.3f})")
+    return out
+
+
 def run_serving(eng, d, qps_levels, n_requests):
     """Poisson arrivals at fixed QPS, lengths logN(mean 400, sigma 0.7) clipped
     to [30, 5000], top-20 per request, the database resident; p50/p99 latency
@@ -279,6 +306,7 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-query batched run")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving run")
     ap.add_argument("--no-dbfile", action="store_true", help="skip the database-file run")
+    ap.add_argument("--no-hybrid", action="store_true", help="skip the CPU + GPU pipeline run")
     ap.add_argument("--serve-qps", type=str, default="5,20,50")
     ap.add_argument("--serve-requests", type=int, default=60)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -374,6 +402,8 @@ def main():
         out["batched_small_db"] = run_small_db(eng)
     if not args.no_dbfile:
         out["dbfile"] = run_dbfile(eng)
+    if not args.no_hybrid:
+        out["hybrid"] = run_hybrid(db)
     if not args.no_serve:
         out["served"] = run_serving(eng, d, [float(x) for x in args.serve_qps.split(",")],
                                     args.serve_requests)

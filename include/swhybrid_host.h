@@ -36,6 +36,36 @@ int swhy_pool_search(const uint8_t* residues, const uint64_t* offsets, const uin
                      int64_t* out_top_pool_index, int32_t* out_scores, double* out_seconds,
                      double* out_gcups);
 
+/* ---- Hybrid pool: HIPWorkers and a CPU worker on one DataPool ----------------
+ * The reference's hybrid scheduling (src/WorkerPool.cpp:25-78): every worker
+ * pulls chunks from the same DataPool, so the faster device takes more of
+ * them.  The CPU worker scans with the host's AVX2 units (int8, then int16
+ * for overflowed lanes); sequences beyond int16 go to the RecalcWorker and
+ * are rescored exactly on device 0. */
+typedef struct swhy_pool_opts {
+    int topk;
+    int hip_workers;           /* HIPWorkers on devices 0..hip_workers-1 */
+    int cpu_threads;           /* threads of the CPU worker; 0 = no CPU worker */
+    uint64_t chunk_bytes;      /* HIPWorker chunk (0 = 256 MB) */
+    uint64_t cpu_chunk_bytes;  /* CPU worker chunk (0 = 16 MB) */
+} swhy_pool_opts;
+
+typedef struct swhy_pool_report {
+    double seconds;            /* first worker start to last stop (recalc included) */
+    double gcups;              /* padded residues x query length / seconds */
+    uint64_t hip_residues, cpu_residues;   /* padded residues each kind searched */
+    double hip_busy_s, cpu_busy_s;         /* time each kind spent computing */
+    uint64_t recalc;           /* sequences rescored after the workers */
+} swhy_pool_report;
+
+/* swhy_pool_search with a CPU worker beside the HIPWorkers.  out_top_* get
+ * opts->topk entries, caller indices; out_scores (optional) every score. */
+int swhy_pool_search_ex(const uint8_t* residues, const uint64_t* offsets,
+                        const uint32_t* lengths, uint64_t nseq, const uint8_t* query, int qlen,
+                        const int32_t* matrix, int gap_open, int gap_extend,
+                        const swhy_pool_opts* opts, int32_t* out_top_score,
+                        int64_t* out_top_index, int32_t* out_scores, swhy_pool_report* report);
+
 /* ---- The reference's on-disk database -------------------------------------
  * <base>.seq / .map / .title as the reference's DBmaker writes them
  * (src/FastaFile.cpp:72-359) and its DataPool reads them
@@ -71,6 +101,12 @@ int swhy_dbfile_pool_search(swhy_dbfile* f, const uint8_t* query, int qlen,
                             uint64_t chunk_bytes, int n_workers, int32_t* out_top_score,
                             int64_t* out_top_index, int32_t* out_scores, double* out_seconds,
                             double* out_gcups);
+/* The same with a CPU worker beside the HIPWorkers (swhy_pool_search_ex). */
+int swhy_dbfile_pool_search_ex(swhy_dbfile* f, const uint8_t* query, int qlen,
+                               const int32_t* matrix, int gap_open, int gap_extend,
+                               const swhy_pool_opts* opts, int32_t* out_top_score,
+                               int64_t* out_top_index, int32_t* out_scores,
+                               swhy_pool_report* report);
 
 /* ---- FASTA queries ------------------------------------------------------------
  * Every record of a FASTA file (the reference reads the first only,

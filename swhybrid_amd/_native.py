@@ -168,7 +168,22 @@ def lib():
 HOST_EXPORTS = ("swhy_last_error", "swhy_pool_search", "swhy_dbfile_make", "swhy_dbfile_open",
                 "swhy_dbfile_info", "swhy_dbfile_arrays", "swhy_dbfile_title",
                 "swhy_dbfile_close", "swhy_dbfile_pool_search", "swhy_fasta_open",
-                "swhy_fasta_count", "swhy_fasta_record", "swhy_fasta_close")
+                "swhy_fasta_count", "swhy_fasta_record", "swhy_fasta_close",
+                "swhy_pool_search_ex", "swhy_dbfile_pool_search_ex")
+
+
+class SwhyPoolOpts(C.Structure):
+    _fields_ = [("topk", C.c_int), ("hip_workers", C.c_int), ("cpu_threads", C.c_int),
+                ("chunk_bytes", C.c_uint64), ("cpu_chunk_bytes", C.c_uint64)]
+
+
+class SwhyPoolReport(C.Structure):
+    _fields_ = [("seconds", C.c_double), ("gcups", C.c_double), ("hip_residues", C.c_uint64),
+                ("cpu_residues", C.c_uint64), ("hip_busy_s", C.c_double),
+                ("cpu_busy_s", C.c_double), ("recalc", C.c_uint64)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
 _host = None
 
 
@@ -203,6 +218,12 @@ def host_lib():
     L.swhy_fasta_record.argtypes = [vp, C.c_uint64, C.POINTER(vp), u64p,
                                     C.POINTER(C.c_char_p)]
     L.swhy_fasta_close.argtypes = [vp]
+    L.swhy_pool_search_ex.argtypes = [vp, vp, vp, C.c_uint64, vp, C.c_int, vp, C.c_int, C.c_int,
+                                      C.POINTER(SwhyPoolOpts), vp, vp, vp,
+                                      C.POINTER(SwhyPoolReport)]
+    L.swhy_dbfile_pool_search_ex.argtypes = [vp, vp, C.c_int, vp, C.c_int, C.c_int,
+                                             C.POINTER(SwhyPoolOpts), vp, vp, vp,
+                                             C.POINTER(SwhyPoolReport)]
     for name in HOST_EXPORTS[1:]:
         getattr(L, name).restype = C.c_int
     _host = L

@@ -25,9 +25,7 @@ class HIPWorker : public Worker {
     ~HIPWorker() override;
     void setQuery(const char* query, size_t qlen) override;
 
-    // host sink for every per-sequence score (pool order), for parity checks
-    void setScoreSink(int* sink) { sink_ = sink; }
-    double kernelSeconds() const { return kernelMs_ / 1e3; }
+    double busySeconds() const override { return kernelMs_ / 1e3; }
     uint64_t cells() const { return cells_; }
 
   private:
@@ -45,7 +43,6 @@ class HIPWorker : public Worker {
     std::vector<int> scores_[2];
     std::vector<uint64_t> offsets_;
     std::vector<uint32_t> lengths_;
-    int* sink_ = nullptr;
     double kernelMs_ = 0;
     uint64_t cells_ = 0;
 };

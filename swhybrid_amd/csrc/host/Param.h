@@ -8,8 +8,9 @@
 
 class Param {
   public:
-    Param(const int* plain32x32, int gapOpen, int gapExtend, int topNum, int hipNum)
-        : cpuNum(0), hipNum(hipNum), gapOpen(gapOpen), gapExtend(gapExtend),
+    Param(const int* plain32x32, int gapOpen, int gapExtend, int topNum, int hipNum,
+          int cpuNum = 0)
+        : cpuNum(cpuNum), hipNum(hipNum), gapOpen(gapOpen), gapExtend(gapExtend),
           gapOE(gapOpen + gapExtend), topNum(topNum) {
         for (int i = 0; i < 32; ++i)
             for (int j = 0; j < 32; ++j) {
@@ -17,8 +18,8 @@ class Param {
                 plain_matrix[i * 32 + j] = plain32x32[i * 32 + j];
             }
     }
-    int cpuNum;
-    int hipNum;
+    int cpuNum;   // threads of the CPU worker (0 = none)
+    int hipNum;   // HIPWorkers, devices 0..hipNum-1
     int gapOpen;
     int gapExtend;
     int gapOE;

@@ -51,6 +51,7 @@ void RecalcWorker::launch() {
             throw g_msg.c_str();
         }
         for (size_t i = 0; i < idx_.size(); ++i) result_.push(sc[i], idx_[i]);
+        scores_ = sc;
         swh_db_destroy(db);
         swh_ctx_destroy(ctx);
     }

@@ -7,7 +7,7 @@
 // int16 with gaps fixed at 12/2 (src/AVXKernel.cpp:914-915); this one runs the
 // exact int32-capable GPU search of libswh on the collected sequences with the
 // search's own scoring.  HIPWorker never pushes here (it rescores overflowed
-// lanes on the device); CPU-side workers would.
+// lanes on the device); the CPU worker pushes what overflows its int16 pass.
 #pragma once
 #include <cstdint>
 #include <mutex>
@@ -24,6 +24,9 @@ class RecalcWorker {
     void pushOverflowSeq(const char* seq, size_t len, size_t idx);
     void launch();
     size_t getNum() const { return idx_.size(); }
+    // after launch(): pool index and exact score of every rescored sequence
+    const std::vector<size_t>& indices() const { return idx_; }
+    const std::vector<int32_t>& scores() const { return scores_; }
     ResultList<int>& getResult() { return result_; }
     double getStartTime() const { return t0_; }
     double getStopTime() const { return t1_; }
@@ -36,6 +39,7 @@ class RecalcWorker {
     std::vector<uint64_t> offsets_;
     std::vector<uint32_t> lengths_;
     std::vector<size_t> idx_;
+    std::vector<int32_t> scores_;
     ResultList<int> result_;
     std::mutex mu_;
     double t0_ = 0, t1_ = 0;

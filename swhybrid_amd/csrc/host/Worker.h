@@ -70,6 +70,10 @@ class Worker {
     double getStopTime() const { return stopTime_; }
     size_t getResidueCount() const { return TotalAminoAcidResidue; }
     const std::string& lastError() const { return error_; }
+    // host sink for every per-sequence score (pool order), for parity checks
+    void setScoreSink(int* sink) { sink_ = sink; }
+    // seconds spent computing (device kernels or host scan)
+    virtual double busySeconds() const { return 0; }
 
     int startThread();
     void waitForThread();
@@ -123,6 +127,7 @@ class Worker {
     const int packSize;
     const int batchSize;
     const int typeSize;
+    int* sink_ = nullptr;
 
   private:
     static void* threadEntry(void* self);

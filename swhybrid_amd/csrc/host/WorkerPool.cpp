@@ -2,12 +2,19 @@
 
 #include <algorithm>
 
-WorkerPool::WorkerPool(DataPool* loader, Param* params, size_t chunkBytes)
+WorkerPool::WorkerPool(DataPool* loader, Param* params, size_t chunkBytes, size_t cpuChunkBytes)
     : loader_(loader), params_(params), resultList_(params->topNum) {
     recalcWorker_ = new RecalcWorker(params, 0);
     recalcWorker_->setQuery(loader->getQuery(), loader->getQueryLen());
-    for (int i = 0; i < params->hipNum; ++i)
+    for (int i = 0; i < params->hipNum; ++i) {
         workerList_.push_back(new HIPWorker(loader, params, recalcWorker_, i, chunkBytes));
+        isCpu_.push_back(false);
+    }
+    if (params->cpuNum > 0) {
+        workerList_.push_back(new CPUWorker(loader, params, recalcWorker_, params->cpuNum,
+                                            cpuChunkBytes ? cpuChunkBytes : (size_t(16) << 20)));
+        isCpu_.push_back(true);
+    }
 }
 
 WorkerPool::~WorkerPool() {
@@ -21,7 +28,7 @@ int WorkerPool::run(int* scoreSink) {
         return -1;
     }
     for (Worker* w : workerList_)
-        if (scoreSink) static_cast<HIPWorker*>(w)->setScoreSink(scoreSink);
+        if (scoreSink) w->setScoreSink(scoreSink);
     for (Worker* w : workerList_) w->startThread();
     for (Worker* w : workerList_) w->waitForThread();
     for (Worker* w : workerList_)
@@ -47,11 +54,29 @@ int WorkerPool::run(int* scoreSink) {
                  : 0;
     for (Worker* w : workerList_) resultList_.mergeList(w->getResult());
     resultList_.mergeList(recalcWorker_->getResult());
+    if (scoreSink)
+        for (size_t i = 0; i < recalcWorker_->indices().size(); ++i)
+            scoreSink[recalcWorker_->indices()[i]] = recalcWorker_->scores()[i];
     return 0;
+}
+
+uint64_t WorkerPool::residues(bool cpu) const {
+    uint64_t r = 0;
+    for (size_t i = 0; i < workerList_.size(); ++i)
+        if (isCpu_[i] == cpu) r += workerList_[i]->getResidueCount();
+    return r;
+}
+
+double WorkerPool::busySeconds(bool cpu) const {
+    double s = 0;
+    for (size_t i = 0; i < workerList_.size(); ++i)
+        if (isCpu_[i] == cpu) s += workerList_[i]->busySeconds();
+    return s;
 }
 
 double WorkerPool::kernelSeconds() const {
     double s = 0;
-    for (Worker* w : workerList_) s += static_cast<HIPWorker*>(w)->kernelSeconds();
+    for (size_t i = 0; i < workerList_.size(); ++i)
+        if (!isCpu_[i]) s += workerList_[i]->busySeconds();
     return s;
 }

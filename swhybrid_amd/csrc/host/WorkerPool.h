@@ -12,6 +12,7 @@
 #include <string>
 #include <vector>
 
+#include "CPUWorker.h"
 #include "DataPool.h"
 #include "HIPWorker.h"
 #include "Param.h"
@@ -20,7 +21,11 @@
 
 class WorkerPool {
   public:
-    WorkerPool(DataPool* loader, Param* params, size_t chunkBytes);
+    // params->hipNum HIPWorkers (chunks of chunkBytes) and, when
+    // params->cpuNum > 0, one CPUWorker with that many threads (chunks of
+    // cpuChunkBytes), in that order (the reference creates the CPU worker
+    // first, src/WorkerPool.cpp:47-52; the order only decides ties).
+    WorkerPool(DataPool* loader, Param* params, size_t chunkBytes, size_t cpuChunkBytes = 0);
     ~WorkerPool();
     // Start every worker, wait for them, rescore, merge.  0 on success.
     int run(int* scoreSink);
@@ -28,6 +33,10 @@ class WorkerPool {
     double seconds() const { return seconds_; }
     double gcups() const { return gcups_; }
     double kernelSeconds() const;
+    // per kind: padded residues searched and seconds spent computing
+    uint64_t residues(bool cpu) const;
+    double busySeconds(bool cpu) const;
+    uint64_t recalcCount() const { return recalcWorker_->getNum(); }
     const std::string& error() const { return error_; }
 
   private:
@@ -35,6 +44,7 @@ class WorkerPool {
     Param* params_;
     RecalcWorker* recalcWorker_;
     std::vector<Worker*> workerList_;
+    std::vector<bool> isCpu_;
     ResultList<int> resultList_;
     double seconds_ = 0, gcups_ = 0;
     std::string error_;

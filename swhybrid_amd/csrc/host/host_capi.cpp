@@ -25,27 +25,32 @@ namespace {
 // Runs the worker pipeline over pool; top-k and scores by pool index
 // translated through pool.originalIndex.
 int run_pool(DataPool& pool, uint64_t nseq, const int32_t* matrix, int gap_open, int gap_extend,
-             int topk, uint64_t chunk_bytes, int n_workers, int32_t* out_top_score,
-             int64_t* out_top_index, int64_t* out_top_pool_index, int32_t* out_scores,
-             double* out_seconds, double* out_gcups) {
+             const swhy_pool_opts& o, int32_t* out_top_score, int64_t* out_top_index,
+             int64_t* out_top_pool_index, int32_t* out_scores, swhy_pool_report* rep) {
     std::vector<int> m(matrix, matrix + 1024);
-    Param params(m.data(), gap_open, gap_extend, topk, n_workers);
+    Param params(m.data(), gap_open, gap_extend, o.topk, o.hip_workers, o.cpu_threads);
     std::vector<int> sink;
     if (out_scores) sink.assign(nseq, -1);
-    double secs = 0, gcups = 0;
     std::vector<Result<int>> top;
+    swhy_pool_report r{};
     {
-        WorkerPool wp(&pool, &params, chunk_bytes ? chunk_bytes : (size_t(256) << 20));
+        WorkerPool wp(&pool, &params, o.chunk_bytes ? o.chunk_bytes : (size_t(256) << 20),
+                      o.cpu_chunk_bytes);
         const int rc = wp.run(out_scores ? sink.data() : nullptr);
         if (rc != 0) {
             g_err = "worker pipeline: " + wp.error();
             return -2;
         }
-        secs = wp.seconds();
-        gcups = wp.gcups();
+        r.seconds = wp.seconds();
+        r.gcups = wp.gcups();
+        r.hip_residues = wp.residues(false);
+        r.cpu_residues = wp.residues(true);
+        r.hip_busy_s = wp.busySeconds(false);
+        r.cpu_busy_s = wp.busySeconds(true);
+        r.recalc = wp.recalcCount();
         top = wp.result().getResult();
     }
-    for (int t = 0; t < topk; ++t) {
+    for (int t = 0; t < o.topk; ++t) {
         const bool real = t < (int)top.size() && top[t].score > 0;
         out_top_score[t] = t < (int)top.size() ? top[t].score : 0;
         out_top_index[t] = real ? (int64_t)pool.originalIndex(top[t].idx) : 0;
@@ -53,9 +58,13 @@ int run_pool(DataPool& pool, uint64_t nseq, const int32_t* matrix, int gap_open,
     }
     if (out_scores)
         for (uint64_t p = 0; p < nseq; ++p) out_scores[pool.originalIndex(p)] = sink[p];
-    if (out_seconds) *out_seconds = secs;
-    if (out_gcups) *out_gcups = gcups;
+    if (rep) *rep = r;
     return 0;
+}
+
+bool bad_opts(const swhy_pool_opts* o, int32_t* ts, int64_t* ti) {
+    return !o || o->topk < 0 || o->hip_workers < 0 || o->cpu_threads < 0 ||
+           o->hip_workers + o->cpu_threads == 0 || (o->topk > 0 && (!ts || !ti));
 }
 
 }  // namespace
@@ -87,9 +96,13 @@ int swhy_pool_search(const uint8_t* residues, const uint64_t* offsets, const uin
     }
     try {
         DataPool pool(residues, offsets, lengths, nseq, query, (size_t)qlen);
-        return run_pool(pool, nseq, matrix, gap_open, gap_extend, topk, chunk_bytes, n_workers,
-                        out_top_score, out_top_index, out_top_pool_index, out_scores,
-                        out_seconds, out_gcups);
+        const swhy_pool_opts o{topk, n_workers, 0, chunk_bytes, 0};
+        swhy_pool_report r{};
+        const int rc = run_pool(pool, nseq, matrix, gap_open, gap_extend, o, out_top_score,
+                                out_top_index, out_top_pool_index, out_scores, &r);
+        if (out_seconds) *out_seconds = r.seconds;
+        if (out_gcups) *out_gcups = r.gcups;
+        return rc;
     } catch (const char* e) {
         g_err = std::string("swhy_pool_search: ") + e;
     } catch (const std::exception& e) {
@@ -190,13 +203,60 @@ int swhy_dbfile_pool_search(swhy_dbfile* h, const uint8_t* query, int qlen,
     }
     try {
         DataPool pool(*h->f, query, (size_t)qlen);
-        return run_pool(pool, h->f->slots(), matrix, gap_open, gap_extend, topk, chunk_bytes,
-                        n_workers, out_top_score, out_top_index, nullptr, out_scores,
-                        out_seconds, out_gcups);
+        const swhy_pool_opts o{topk, n_workers, 0, chunk_bytes, 0};
+        swhy_pool_report r{};
+        const int rc = run_pool(pool, h->f->slots(), matrix, gap_open, gap_extend, o,
+                                out_top_score, out_top_index, nullptr, out_scores, &r);
+        if (out_seconds) *out_seconds = r.seconds;
+        if (out_gcups) *out_gcups = r.gcups;
+        return rc;
     } catch (const char* e) {
         g_err = std::string("swhy_dbfile_pool_search: ") + e;
     } catch (const std::exception& e) {
         g_err = std::string("swhy_dbfile_pool_search: ") + e.what();
+    }
+    return -2;
+}
+
+int swhy_pool_search_ex(const uint8_t* residues, const uint64_t* offsets,
+                        const uint32_t* lengths, uint64_t nseq, const uint8_t* query, int qlen,
+                        const int32_t* matrix, const int gap_open, int gap_extend,
+                        const swhy_pool_opts* opts, int32_t* out_top_score,
+                        int64_t* out_top_index, int32_t* out_scores, swhy_pool_report* report) {
+    if (!query || qlen <= 0 || !matrix || (nseq > 0 && (!residues || !offsets || !lengths)) ||
+        bad_opts(opts, out_top_score, out_top_index)) {
+        g_err = "swhy_pool_search_ex: bad arguments";
+        return -1;
+    }
+    try {
+        DataPool pool(residues, offsets, lengths, nseq, query, (size_t)qlen);
+        return run_pool(pool, nseq, matrix, gap_open, gap_extend, *opts, out_top_score,
+                        out_top_index, nullptr, out_scores, report);
+    } catch (const char* e) {
+        g_err = std::string("swhy_pool_search_ex: ") + e;
+    } catch (const std::exception& e) {
+        g_err = std::string("swhy_pool_search_ex: ") + e.what();
+    }
+    return -2;
+}
+
+int swhy_dbfile_pool_search_ex(swhy_dbfile* h, const uint8_t* query, int qlen,
+                               const int32_t* matrix, int gap_open, int gap_extend,
+                               const swhy_pool_opts* opts, int32_t* out_top_score,
+                               int64_t* out_top_index, int32_t* out_scores,
+                               swhy_pool_report* report) {
+    if (!h || !query || qlen <= 0 || !matrix || bad_opts(opts, out_top_score, out_top_index)) {
+        g_err = "swhy_dbfile_pool_search_ex: bad arguments";
+        return -1;
+    }
+    try {
+        DataPool pool(*h->f, query, (size_t)qlen);
+        return run_pool(pool, h->f->slots(), matrix, gap_open, gap_extend, *opts, out_top_score,
+                        out_top_index, nullptr, out_scores, report);
+    } catch (const char* e) {
+        g_err = std::string("swhy_dbfile_pool_search_ex: ") + e;
+    } catch (const std::exception& e) {
+        g_err = std::string("swhy_dbfile_pool_search_ex: ") + e.what();
     }
     return -2;
 }

@@ -2,10 +2,11 @@
 // reference's flags and printout (src/Param.h:59-117, src/main.cpp:7-16,
 // src/WorkerPool.cpp:107-155):
 //   swhybrid -db <base> -q <queries.fasta> [-t top] [-o open] [-e extend]
-//            [-g hip_workers] [-m blosum62|blosum50] [-batch]
+//            [-g hip_workers] [-c cpu_threads] [-m blosum62|blosum50] [-batch]
 // One query record runs through the worker pipeline (DataPool over the
-// database files -> HIPWorker); several records, or -batch, are searched in
-// one persistent launch over the database held resident in HBM.
+// database files -> HIPWorkers, and a CPU worker when -c > 0); several
+// records, or -batch, are searched in one persistent launch over the
+// database held resident in HBM.
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -22,7 +23,7 @@ namespace {
 struct Args {
     const char* db = nullptr;
     const char* query = nullptr;
-    int top = 10, open = 10, extend = 2, gpus = 1;
+    int top = 10, open = 10, extend = 2, gpus = 1, cpus = 0;
     const char* matrix = "blosum62";
     bool batch = false;
 };
@@ -30,8 +31,8 @@ struct Args {
 [[noreturn]] void usage() {
     std::fprintf(stderr,
                  "usage: swhybrid -db <protein database base> -q <query FASTA> [-t top] "
-                 "[-o gap open] [-e gap extend] [-g HIP workers] [-m blosum62|blosum50] "
-                 "[-batch]\n");
+                 "[-o gap open] [-e gap extend] [-g HIP workers] [-c CPU worker threads] "
+                 "[-m blosum62|blosum50] [-batch]\n");
     std::exit(2);
 }
 
@@ -51,10 +52,12 @@ Args parse(int argc, char** argv) {
         else if (f == "-g") a.gpus = std::atoi(val());
         else if (f == "-m") a.matrix = val();
         else if (f == "-batch") a.batch = true;
-        else if (f == "-c" || f == "-p") (void)val();   // CPU / Xeon Phi workers: none here
+        else if (f == "-c") a.cpus = std::atoi(val());
+        else if (f == "-p") (void)val();   // Xeon Phi workers: none here
         else usage();
     }
-    if (!a.db || !a.query || a.top < 1 || a.gpus < 1) usage();
+    if (!a.db || !a.query || a.top < 1 || a.gpus < 0 || a.cpus < 0 || a.gpus + a.cpus == 0)
+        usage();
     return a;
 }
 
@@ -106,13 +109,18 @@ int main(int argc, char** argv) {
     std::vector<int64_t> ti(a.top);
     if (nq == 1 && !a.batch) {
         std::printf("Query sequence length %d\n", qlens[0]);
-        double secs = 0, gcups = 0;
-        if (swhy_dbfile_pool_search(db, qs[0], qlens[0], matrix, a.open, a.extend, a.top, 0,
-                                    a.gpus, ts.data(), ti.data(), nullptr, &secs, &gcups) != 0)
+        const swhy_pool_opts o{a.top, a.gpus, a.cpus, 0, 0};
+        swhy_pool_report rep{};
+        if (swhy_dbfile_pool_search_ex(db, qs[0], qlens[0], matrix, a.open, a.extend, &o,
+                                       ts.data(), ti.data(), nullptr, &rep) != 0)
             return fail("search", swhy_last_error());
         print_hits(db, ts.data(), ti.data(), a.top);
-        std::printf("total calculation time: %lf, GCUPS %lf\n, total residue %llu\n", secs,
-                    gcups, (unsigned long long)residues);
+        std::printf("total calculation time: %lf, GCUPS %lf\n, total residue %llu\n",
+                    rep.seconds, rep.gcups, (unsigned long long)residues);
+        if (a.cpus > 0)
+            std::printf("residues searched: HIP %llu, CPU %llu; rescored after the workers %llu\n",
+                        (unsigned long long)rep.hip_residues, (unsigned long long)rep.cpu_residues,
+                        (unsigned long long)rep.recalc);
     } else {
         const uint8_t* res = nullptr;
         const uint64_t* off = nullptr;
@@ -120,6 +128,7 @@ int main(int argc, char** argv) {
         swhy_dbfile_arrays(db, &res, &off, &len);
         swh_ctx* ctx = nullptr;
         swh_db* d = nullptr;
+        if (a.gpus < 1) return fail("-batch", "needs a GPU (-g 1)");
         if (swh_ctx_create(0, matrix, a.open, a.extend, &ctx) != SWH_OK)
             return fail("device", swh_last_error());
         if (swh_db_create(ctx, res, off, len, slots, nullptr, &d) != SWH_OK)

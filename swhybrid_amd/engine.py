@@ -266,3 +266,30 @@ def pool_search(db: SeqDB, query, matrix="blosum62", gap_open=10, gap_extend=2, 
         raise N.SwhError(f"pool search failed ({rc}): {msg.decode() if msg else ''}")
     return {"top_score": ts, "top_index": ti, "top_pool_index": tp, "scores": sc,
             "seconds": secs.value, "gcups": gc.value}
+
+
+def hybrid_search(db: SeqDB, query, matrix="blosum62", gap_open=10, gap_extend=2, k=10,
+                  hip_workers=1, cpu_threads=0, chunk_bytes=0, cpu_chunk_bytes=0,
+                  all_scores=False):
+    """The worker pipeline with HIPWorkers and a CPU worker (AVX2, cpu_threads
+    threads) pulling chunks from one DataPool.  Returns dict(top_score,
+    top_index, scores, report)."""
+    m = _as_matrix(matrix)
+    q = _as_query(query)
+    res = np.ascontiguousarray(db.residues, dtype=np.uint8)
+    off = np.ascontiguousarray(db.offsets, dtype=np.uint64)
+    lens = np.ascontiguousarray(db.lengths, dtype=np.uint32)
+    ts = np.zeros(k, dtype=np.int32)
+    ti = np.zeros(k, dtype=np.int64)
+    sc = np.zeros(db.n, dtype=np.int32) if all_scores else None
+    o = N.SwhyPoolOpts(int(k), int(hip_workers), int(cpu_threads), int(chunk_bytes),
+                       int(cpu_chunk_bytes))
+    rep = N.SwhyPoolReport()
+    L = N.host_lib()
+    rc = L.swhy_pool_search_ex(N.ptr(res), N.ptr(off), N.ptr(lens), db.n, N.ptr(q), len(q),
+                               N.ptr(m), int(gap_open), int(gap_extend), C.byref(o), N.ptr(ts),
+                               N.ptr(ti), None if sc is None else N.ptr(sc), C.byref(rep))
+    if rc != 0:
+        msg = L.swhy_last_error()
+        raise N.SwhError(f"hybrid search failed ({rc}): {msg.decode() if msg else ''}")
+    return {"top_score": ts, "top_index": ti, "scores": sc, "report": rep.as_dict()}
