@@ -218,11 +218,12 @@ def run_dbfile(eng):
 def run_hybrid(db, qlens=(1000, 5478)):
     """The reference's hybrid scheduling: the worker pipeline over the same
     5M-sequence database held in host memory, chunks of 256 MB to the
-    HIPWorker and of 16 MB to the CPU worker (the host's AVX2 cores), all
+    HIPWorker and of 4 MB to the CPU worker (the host's AVX2 cores), all
     from one DataPool; HIPWorker alone first, then both."""
     from swhybrid_amd import datagen as G
     from swhybrid_amd.engine import hybrid_search
-    threads = min(16, os.cpu_count() or 1)
+    # the HIPWorker's own packing uses up to 16 host threads: leave it 4
+    threads = max(1, min(16, os.cpu_count() or 1) - 4)
     out = {"workload": "worker pipeline over the 5M-sequence database in host memory: "
                        "HIPWorker alone vs HIPWorker + CPU worker on one DataPool",
            "cpu_threads": threads}

@@ -47,7 +47,8 @@ typedef struct swhy_pool_opts {
     int hip_workers;           /* HIPWorkers on devices 0..hip_workers-1 */
     int cpu_threads;           /* threads of the CPU worker; 0 = no CPU worker */
     uint64_t chunk_bytes;      /* HIPWorker chunk (0 = 256 MB) */
-    uint64_t cpu_chunk_bytes;  /* CPU worker chunk (0 = 16 MB) */
+    uint64_t cpu_chunk_bytes;  /* CPU worker chunk (0 = 4 MB); the CPU worker takes no
+                                  chunk once only hip_workers x chunk_bytes are left */
 } swhy_pool_opts;
 
 typedef struct swhy_pool_report {

@@ -49,34 +49,45 @@ size_t DataPool::maxInfoPerChunk(size_t bufferSize) const {
 void DataPool::rewind() {
     std::lock_guard<std::mutex> g(mu_);
     next_ = 0;
+    handedOut_ = 0;
 }
 
 bool DataPool::fillChunk(char* dst, batchInfo* info, size_t& infoSize, size_t& aaSize,
-                         size_t bufferSize, bool /*cpuFlag*/, size_t& baseIndex) {
-    std::lock_guard<std::mutex> g(mu_);
+                         size_t bufferSize, bool cpuFlag, size_t& baseIndex) {
     infoSize = 0;
     aaSize = 0;
-    if (next_ >= batches_.size()) return false;
-    baseIndex = batches_[next_].first;
+    size_t b0, b1;
+    {
+        // claim a run of batches under the lock, copy it outside
+        std::lock_guard<std::mutex> g(mu_);
+        if (next_ >= batches_.size()) return false;
+        if (cpuFlag && totalPadded_ - handedOut_ <= cpuReserve_) return false;
+        b0 = next_;
+        size_t used = 0;
+        while (next_ < batches_.size()) {
+            const Batch& b = batches_[next_];
+            const size_t bytes = (size_t)b.num * b.len;
+            if (used + bytes > bufferSize && next_ > b0) break;
+            if (bytes > bufferSize) throw "error: readBufferSize too small\n";
+            used += bytes;
+            ++next_;
+        }
+        b1 = next_;
+        handedOut_ += used;
+    }
+    baseIndex = batches_[b0].first;
     size_t used = 0;
-    while (next_ < batches_.size()) {
-        const Batch& b = batches_[next_];
-        const size_t bytes = (size_t)b.num * b.len;
-        if (used + bytes > bufferSize && infoSize > 0) break;
-        if (bytes > bufferSize) throw "error: readBufferSize too small\n";
+    for (size_t bi = b0; bi < b1; ++bi) {
+        const Batch& b = batches_[bi];
         char* out = dst + used;
         for (unsigned k = 0; k < b.num; ++k) {
             const uint32_t id = order_[b.first + k];
-            const uint32_t n = lengths_[id];
-            const uint8_t* src = residues_ + offsets_[id];
-            for (uint32_t r = 0; r < b.len; ++r) {
-                uint8_t c = r < n ? src[r] : (uint8_t)kPadCode;
-                out[(size_t)k * b.len + r] = (char)(c > kPadCode ? kPadCode : c);
-            }
+            const uint32_t n = std::min<uint32_t>(lengths_[id], b.len);
+            std::memcpy(out + (size_t)k * b.len, residues_ + offsets_[id], n);
+            std::memset(out + (size_t)k * b.len + n, kPadCode, b.len - n);
         }
         info[infoSize++] = batchInfo(b.num, b.len);
-        used += bytes;
-        ++next_;
+        used += (size_t)b.num * b.len;
     }
     aaSize = used;
     return true;

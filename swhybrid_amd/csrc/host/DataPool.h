@@ -24,8 +24,10 @@ class DataPool {
     // Pool index = file index (fillers included).
     DataPool(const swhy::DbFile& db, const uint8_t* query, size_t qlen);
 
-    // Copies the next batches into dst while they fit in bufferSize bytes.
-    // Returns false (infoSize = 0) when the pool is exhausted.
+    // Copies the next batches into dst while they fit in bufferSize bytes
+    // (codes as given: consumers clamp codes above 23 when they pack).
+    // Returns false (infoSize = 0) when the pool is exhausted, or, for a CPU
+    // worker (cpuFlag), once no more than the CPU reserve is left.
     bool fillChunk(char* dst, batchInfo* seqBatchInfo, size_t& infoSize, size_t& aaSize,
                    size_t bufferSize, bool cpuFlag, size_t& baseIndex);
 
@@ -41,6 +43,9 @@ class DataPool {
     // database.
     uint64_t originalIndex(size_t poolIdx) const { return order_[poolIdx]; }
     void rewind();
+    // Residue bytes at the end of the pool that CPU workers leave to the
+    // device workers, so that a slow CPU chunk does not finish last.
+    void setCpuReserve(size_t bytes) { cpuReserve_ = bytes; }
 
   private:
     struct Batch {
@@ -55,6 +60,8 @@ class DataPool {
     std::vector<Batch> batches_;
     std::vector<uint8_t> query_;
     size_t next_ = 0;
+    size_t handedOut_ = 0;
+    size_t cpuReserve_ = 0;
     size_t totalPadded_ = 0;
     size_t maxBatchBytes_ = 0;
     std::mutex mu_;

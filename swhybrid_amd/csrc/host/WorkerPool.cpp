@@ -12,8 +12,10 @@ WorkerPool::WorkerPool(DataPool* loader, Param* params, size_t chunkBytes, size_
     }
     if (params->cpuNum > 0) {
         workerList_.push_back(new CPUWorker(loader, params, recalcWorker_, params->cpuNum,
-                                            cpuChunkBytes ? cpuChunkBytes : (size_t(16) << 20)));
+                                            cpuChunkBytes ? cpuChunkBytes : (size_t(4) << 20)));
         isCpu_.push_back(true);
+        // the last device chunk's worth of work is left to the device workers
+        if (params->hipNum > 0) loader->setCpuReserve(chunkBytes * (size_t)params->hipNum);
     }
 }
 

@@ -7,6 +7,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <numeric>
 #include <string>
 #include <thread>
@@ -123,8 +124,10 @@ int swh_db_create(swh_ctx* c, const uint8_t* residues, const uint64_t* offsets,
     // hand out the longest blocks first
     std::vector<uint32_t> order(nseq);
     std::iota(order.begin(), order.end(), 0u);
-    std::stable_sort(order.begin(), order.end(),
-                     [&](uint32_t a, uint32_t b) { return lengths[a] > lengths[b]; });
+    // (a DataPool chunk arrives sorted already)
+    if (!std::is_sorted(lengths, lengths + nseq, std::greater<uint32_t>()))
+        std::stable_sort(order.begin(), order.end(),
+                         [&](uint32_t a, uint32_t b) { return lengths[a] > lengths[b]; });
     uint64_t total = 0;
     uint64_t n_long = 0;
     for (uint64_t i = 0; i < nseq; ++i) {
