@@ -338,15 +338,27 @@ static int tuned_group_lanes(const swh_ctx* c, int qlen, int Wg, const swh_searc
     return 0;
 }
 
+// Built-in cost per padded query column of each lane configuration (order of
+// kLaneCfgs), measured on MI355X over a 5M-sequence database: the pair table
+// at W=24 is 1-1.2% cheaper than at W=16 from query length ~1000 on, but
+// 3-6% dearer below (fewer waves per SIMD hide less of the per-strip set-up);
+// the single-residue profile costs 6-12% more.
+static double builtin_cost(int k, int qlen) {
+    switch (k) {
+        case 0: return 1.0;                           // (16, pair)
+        case 1: return qlen >= 768 ? 0.990 : 1.03;    // (24, pair)
+        case 2: return 1.115;                         // (16, single)
+        case 3: return 1.066;                         // (24, single)
+        default: return 1.056;                        // (32, single)
+    }
+}
+
 // Launch shape shared by the queries of a launch.  Lane scan: the width and
 // profile placement with the least predicted time, summed over the queries as
 // cost per padded query column x padded columns.  Costs come from the tuning
-// table when one is installed, else from built-in weights measured on MI355X
-// (5M-sequence database: pair table at W=16 and W=24 within 0.5%, single
-// profile at W=32 -- no room for a pair table -- 6% slower).  Group scan: the
+// table when one is installed, else from builtin_cost.  Group scan: the
 // width of the bucket holding most of the batch's columns, else 16.
 Shape choose_shape(const swh_ctx* c, const int* qlens, int nq, const swh_search_opts& o) {
-    static const float kBuiltin[SWH_TUNE_LANE_CONFIGS] = {1.0f, 1.003f, 1.02f, 1.02f, 1.065f};
     Shape sh{16, o.group_width, true};
     double best = 1e300;
     for (int k = 0; k < SWH_TUNE_LANE_CONFIGS; ++k) {
@@ -357,7 +369,7 @@ Shape choose_shape(const swh_ctx* c, const int* qlens, int nq, const swh_search_
         double t = 0;
         for (int i = 0; i < nq; ++i) {
             const int b = tune_bucket(c, qlens[i]);
-            double w = kBuiltin[k];
+            double w = builtin_cost(k, qlens[i]);
             if (b >= 0) {
                 const float m = c->tune[b].lane_cost[k];
                 w = m > 0 ? m : 1e30;   // not measured: never preferred

@@ -1,0 +1,110 @@
+"""Parity at the benchmark's full sizes.  The 5M-sequence database and its five
+query lengths are searched in full on the GPU; the oracle checks a sample
+(random sequences plus the longest ones) exactly, and properties that do not
+depend on size check everything: every strip width and the batched launch
+give the same scores, the top-k equals a sort of the scores, and a forced
+int32 rescore changes nothing.  The near-copy database (20% planted copies,
+BLOSUM62 10/2 and BLOSUM50 13/2) runs at a tenth of its size."""
+import numpy as np
+import pytest
+
+import oracle as O
+from swhybrid_amd import alphabet as A
+from swhybrid_amd import datagen as G
+from swhybrid_amd.engine import Engine, topk
+
+pytestmark = pytest.mark.gpu
+
+
+def sample_idx(db, rng, n_random=1500, n_longest=100):
+    longest = np.argsort(-db.lengths.astype(np.int64), kind="stable")[:n_longest]
+    rand = rng.choice(db.n, size=n_random, replace=False)
+    return np.unique(np.concatenate([longest, rand]))
+
+
+def check_sample(q, db, got, idx, matrix, go, ge):
+    want = O.search_i32(q, db.subset(idx), matrix, go, ge)
+    bad = np.flatnonzero(got[idx] != want)
+    assert bad.size == 0, (f"{bad.size} mismatches at {idx[bad[:5]]}: got {got[idx][bad[:5]]} "
+                           f"want {want[bad[:5]]}")
+
+
+def sorted_topk(scores, k):
+    order = np.lexsort((np.arange(scores.size), -scores.astype(np.int64)))[:k]
+    s = scores[order]
+    return np.where(s > 0, s, 0), np.where(s > 0, order, 0)
+
+
+@pytest.fixture(scope="module")
+def c2():
+    db = G.c2_db()
+    e = Engine("blosum62", 10, 2)
+    d = e.load_db(db)
+    yield db, e, d
+    d.close()
+    e.close()
+
+
+def test_c2_all_query_lengths(c2):
+    db, e, d = c2
+    rng = np.random.Generator(np.random.PCG64(7001))
+    idx = sample_idx(db, rng)
+    qs = G.c2_queries()
+    single = []
+    for q in qs:
+        got, st = e.search(d, q)
+        assert st["cells"] == db.total * len(q)
+        check_sample(q, db, got, idx, e.matrix, 10, 2)
+        s1, i1 = topk(got, 20)
+        s2, i2 = sorted_topk(got, 20)
+        assert np.array_equal(s1, s2) and np.array_equal(i1, i2)
+        single.append(got)
+    batch, _ = e.search_batch(d, qs)
+    for i in range(len(qs)):
+        assert np.array_equal(batch[i], single[i]), f"batched query {i}"
+
+
+@pytest.mark.parametrize("width,placement", [(24, "pair"), (32, "single"), (16, "single")])
+def test_c2_widths_agree(c2, width, placement):
+    db, e, d = c2
+    q = G.c2_queries()[2]                              # length 1000
+    ref, _ = e.search(d, q, strip_width=16, profile_mode="pair")
+    got, _ = e.search(d, q, strip_width=width, profile_mode=placement)
+    assert np.array_equal(got, ref)
+
+
+def test_c2_group_layout_agrees(c2):
+    db, e, _ = c2
+    q = G.c2_queries()[3]                              # length 2005
+    d1 = e.load_db(db, group_min_len=600)              # many more sequences in lane groups
+    d2 = e.load_db(db, group_min_len=-1)               # none
+    a, st1 = e.search(d1, q)
+    b, st2 = e.search(d2, q)
+    d1.close()
+    d2.close()
+    assert st1["cells_group"] > st2["cells_group"] == 0
+    assert np.array_equal(a, b)
+
+
+@pytest.fixture(scope="module")
+def c5_tenth():
+    return G.c5_db(n=200_000)
+
+
+@pytest.mark.parametrize("matrix,go,ge", [("blosum62", 10, 2), ("blosum50", 13, 2)])
+def test_near_copies_tenth_scale(c5_tenth, matrix, go, ge):
+    db, qs, planted = c5_tenth
+    e = Engine(matrix, go, ge)
+    d = e.load_db(db)
+    rng = np.random.Generator(np.random.PCG64(7002))
+    idx = np.unique(np.concatenate([sample_idx(db, rng, n_random=400, n_longest=50),
+                                    rng.choice(planted, size=150, replace=False)]))
+    for q in qs:
+        got, st = e.search(d, q)
+        assert got[planted].max() > 10_000                # the copies score high
+        check_sample(q, db, got, idx, e.matrix, go, ge)
+        forced, st2 = e.search(d, q, overflow_threshold=5_000)
+        assert st2["n_overflow"] >= int((got >= 5_000).sum()) > 0
+        assert np.array_equal(forced, got)
+    d.close()
+    e.close()
