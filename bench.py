@@ -222,8 +222,9 @@ def run_hybrid(db, qlens=(1000, 5478)):
     from one DataPool; HIPWorker alone first, then both."""
     from swhybrid_amd import datagen as G
     from swhybrid_amd.engine import hybrid_search
-    # the HIPWorker's own packing uses up to 16 host threads: leave it 4
-    threads = max(1, min(16, os.cpu_count() or 1) - 4)
+    # the HIPWorker's own host work (chunk copies, upload) competes with the
+    # CPU worker's threads: 4 threads is where the CPU costs it least
+    threads = min(4, os.cpu_count() or 1)
     out = {"workload": "worker pipeline over the 5M-sequence database in host memory: "
                        "HIPWorker alone vs HIPWorker + CPU worker on one DataPool",
            "cpu_threads": threads}

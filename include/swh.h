@@ -88,6 +88,12 @@ const char* swh_last_error(void);
 int swh_version(void);
 int swh_device_count(int* count);
 
+/* Page-locked host memory: buffers handed to swh_db_create from it are
+ * uploaded by DMA without a staging copy.  No reference counterpart (the
+ * reference's CUDAWorker pins its chunk buffers, src/CUDAWorker.cu:53-98). */
+int swh_pinned_alloc(size_t bytes, void** out);
+int swh_pinned_free(void* p);
+
 /* Context: device, scoring scheme, search stream and scratch.
  * Replaces CUDAWorker's constructor/alloc (src/CUDAWorker.cu:29-98) and the
  * Param scoring fields it reads (src/Param.h:59-117). */

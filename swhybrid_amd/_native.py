@@ -23,6 +23,7 @@ EXPORTS = (
     "swh_db_info", "swh_search", "swh_search_batch", "swh_topk",
     "swh_server_create", "swh_server_submit", "swh_server_wait", "swh_server_get_stats",
     "swh_server_destroy", "swh_autotune", "swh_set_tuning", "swh_get_tuning",
+    "swh_pinned_alloc", "swh_pinned_free",
 )
 
 SWH_PROFILE_AUTO, SWH_PROFILE_SINGLE, SWH_PROFILE_PAIR = 0, 1, 2
@@ -157,6 +158,8 @@ def lib():
     L.swh_autotune.argtypes = [vp, vp, vp, vp, C.c_uint64, C.c_int, vp, C.c_int, C.c_int,
                                C.c_int, C.POINTER(SwhTuneResult)]
     L.swh_set_tuning.argtypes = [vp, C.POINTER(SwhTuneEntry), C.c_int]
+    L.swh_pinned_alloc.argtypes = [C.c_size_t, C.POINTER(vp)]
+    L.swh_pinned_free.argtypes = [vp]
     L.swh_get_tuning.argtypes = [vp, C.POINTER(SwhTuneEntry), C.c_int, C.POINTER(C.c_int)]
     for name in EXPORTS:
         if name != "swh_last_error" and name != "swh_version":

@@ -1,5 +1,6 @@
 #include "CPUWorker.h"
 
+#include <algorithm>
 #include <atomic>
 #include <chrono>
 #include <cstdio>
@@ -58,19 +59,39 @@ void CPUWorker::search() {
     const std::vector<batchInfo>& bs = batches_[slot];
     const uint8_t* data = reinterpret_cast<const uint8_t*>(buf_[slot].data());
     int* out = scores_[slot].data();
-    // pieces of up to 256 sequences of one batch
+    // every sequence of the chunk, longest first, cut into pieces of whole
+    // 32-lane groups of about 256 KB of residues: batches of one length can
+    // be small (a long tail of distinct lengths), so groups span batches
+    std::vector<const uint8_t*> ptr;
+    std::vector<uint32_t> len;
+    size_t boff = 0;
+    for (const batchInfo& b : bs) {
+        for (uint32_t k = 0; k < b.numSeqs; ++k) {
+            const uint8_t* p = data + boff + (size_t)k * b.seqLen;
+            uint32_t n = b.seqLen;
+            while (n > 0 && p[n - 1] == 23) --n;   // trailing pads cost rows only
+            ptr.push_back(p);
+            len.push_back(n);
+        }
+        boff += (size_t)b.numSeqs * b.seqLen;
+    }
+    const size_t nseq = ptr.size();
+    std::vector<uint32_t> order(nseq);
+    for (size_t i = 0; i < nseq; ++i) order[i] = (uint32_t)i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](uint32_t a, uint32_t b) { return len[a] > len[b]; });
     struct Piece {
-        size_t byte_off, seq_off;
-        uint32_t n, len;
+        size_t first, n;
     };
     std::vector<Piece> pieces;
-    size_t boff = 0, soff = 0;
-    for (const batchInfo& b : bs) {
-        for (uint32_t k = 0; k < b.numSeqs; k += 256)
-            pieces.push_back(Piece{boff + (size_t)k * b.seqLen, soff + k,
-                                   std::min<uint32_t>(256, b.numSeqs - k), b.seqLen});
-        boff += (size_t)b.numSeqs * b.seqLen;
-        soff += b.numSeqs;
+    for (size_t i = 0; i < nseq;) {
+        size_t n = 0, bytes = 0;
+        while (i + n < nseq && (n == 0 || n % 32 != 0 || bytes < (size_t(256) << 10))) {
+            bytes += len[order[i + n]];
+            ++n;
+        }
+        pieces.push_back(Piece{i, n});
+        i += n;
     }
     const uint8_t* q = reinterpret_cast<const uint8_t*>(query.data());
     const size_t base = compEntry->getBaseIdx();
@@ -79,21 +100,29 @@ void CPUWorker::search() {
     auto work = [&]() {
         swhy::cpu::Scratch scratch;
         std::vector<int32_t> sc;
+        std::vector<const uint8_t*> pp;
+        std::vector<uint32_t> pl;
         for (size_t p = next++; p < pieces.size(); p = next++) {
             const Piece& pc = pieces[p];
+            pp.resize(pc.n);
+            pl.resize(pc.n);
+            for (size_t k = 0; k < pc.n; ++k) {
+                pp[k] = ptr[order[pc.first + k]];
+                pl[k] = len[order[pc.first + k]];
+            }
             sc.resize(pc.n);
-            swhy::cpu::score_batch(scoring_, q, queryLen, data + pc.byte_off, pc.n, pc.len,
-                                   sc.data(), scratch);
-            for (uint32_t k = 0; k < pc.n; ++k) {
+            swhy::cpu::score_seqs(scoring_, q, queryLen, pp.data(), pl.data(), (uint32_t)pc.n,
+                                  sc.data(), scratch);
+            for (size_t k = 0; k < pc.n; ++k) {
+                const uint32_t i = order[pc.first + k];
                 int v = sc[k];
                 if (v < 0) {   // beyond int16: exact int32 after the workers finish
-                    recalcWorker->pushOverflowSeq(
-                        reinterpret_cast<const char*>(data + pc.byte_off + (size_t)k * pc.len),
-                        pc.len, base + pc.seq_off + k);
+                    recalcWorker->pushOverflowSeq(reinterpret_cast<const char*>(ptr[i]), len[i],
+                                                  base + i);
                     ++pushed;
                     v = 0;
                 }
-                out[pc.seq_off + k] = v;
+                out[i] = v;
             }
         }
     };

@@ -6,9 +6,12 @@
 //   alloc      checks the host for AVX2, fixes the scoring
 //   packDB     copies the chunk and its batch table into the aux entry (the
 //              chunk buffer is refilled while this entry is searched)
-//   search     a team of threads takes (batch, 256-sequence) pieces from an
-//              atomic counter: int8 pass, int16 pass for the lanes that
-//              overflowed, and sequences beyond int16 go to the RecalcWorker
+//   search     the chunk's sequences, longest first, in pieces of whole
+//              32-lane groups (~256 KB each; groups span batches, whose long
+//              tail holds few sequences of each length), taken by a team of
+//              threads from an atomic counter: int8 pass, int16 pass for the
+//              lanes that overflowed, and sequences beyond int16 go to the
+//              RecalcWorker
 //              (scored 0 here, as the reference's workers do,
 //              src/AVXWorker.cpp:429-459)
 #pragma once

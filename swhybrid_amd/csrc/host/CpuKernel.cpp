@@ -90,13 +90,16 @@ void make_tables(const Scoring& sc, const uint8_t* q, int qlen, Tables& t) {
 }
 
 // Lane l of the group holds sequence idx[l] (fewer than kLanes: pad lanes);
-// inter[row * kLanes + l] = its code at row (23 beyond its end or above 23).
+// inter[row * kLanes + l] = its code at row (23 beyond its end or above 23)
+// for rows [0, rows).
 template <class T>
-void interleave(const uint8_t* seqs, uint32_t len, const uint32_t* idx, int n, uint8_t* inter) {
-    const size_t total = (size_t)len * T::kLanes;
-    if (n < T::kLanes) std::memset(inter, 23, total);
+void interleave(const uint8_t* const* seqs, const uint32_t* lens, uint32_t rows,
+                const uint32_t* idx, int n, uint8_t* inter) {
+    const size_t total = (size_t)rows * T::kLanes;
+    std::memset(inter, 23, total);
     for (int l = 0; l < n; ++l) {
-        const uint8_t* src = seqs + (size_t)idx[l] * len;
+        const uint8_t* src = seqs[idx[l]];
+        const uint32_t len = std::min(lens[idx[l]], rows);
         uint8_t* o = inter + l;
         for (uint32_t r = 0; r < len; ++r) o[(size_t)r * T::kLanes] = src[r];
     }
@@ -157,10 +160,9 @@ __m256i scan_group(const Tables& tb, const uint8_t* q, int m, const uint8_t* int
 }
 
 template <class T>
-void run_groups(const Tables& tb, const uint8_t* q, int m, const uint8_t* seqs, uint32_t len,
-                const uint32_t* idx, size_t n, int gap_oe, int gap_e, int32_t* score_of_idx_pos,
-                Scratch& s) {
-    s.inter.resize((size_t)len * T::kLanes);
+void run_groups(const Tables& tb, const uint8_t* q, int m, const uint8_t* const* seqs,
+                const uint32_t* lens, const uint32_t* idx, size_t n, int gap_oe, int gap_e,
+                int32_t* score_of_idx_pos, Scratch& s) {
     s.colstate.resize((size_t)m * 2 * sizeof(__m256i) + 32);
     s.prof.resize(kSymbols * kRows * sizeof(__m256i) + 32);
     auto align = [](std::vector<uint8_t>& v) {
@@ -171,8 +173,12 @@ void run_groups(const Tables& tb, const uint8_t* q, int m, const uint8_t* seqs, 
     __m256i* prof = align(s.prof);
     for (size_t g = 0; g < n; g += T::kLanes) {
         const int cnt = (int)std::min<size_t>(T::kLanes, n - g);
-        interleave<T>(seqs, len, idx + g, cnt, s.inter.data());
-        const __m256i S = scan_group<T>(tb, q, m, s.inter.data(), len, gap_oe, gap_e, col, prof);
+        uint32_t rows = 0;
+        for (int l = 0; l < cnt; ++l) rows = std::max(rows, lens[idx[g + l]]);
+        rows = std::max<uint32_t>(kRows, (rows + kRows - 1) / kRows * kRows);
+        if (s.inter.size() < (size_t)rows * T::kLanes) s.inter.resize((size_t)rows * T::kLanes);
+        interleave<T>(seqs, lens, rows, idx + g, cnt, s.inter.data());
+        const __m256i S = scan_group<T>(tb, q, m, s.inter.data(), rows, gap_oe, gap_e, col, prof);
         int32_t lanes[T::kLanes];
         T::scores(S, lanes);
         for (int l = 0; l < cnt; ++l) score_of_idx_pos[g + l] = lanes[l];
@@ -181,49 +187,47 @@ void run_groups(const Tables& tb, const uint8_t* q, int m, const uint8_t* seqs, 
 
 }  // namespace
 
-void score_batch(const Scoring& sc, const uint8_t* query, int qlen, const uint8_t* seqs,
-                 uint32_t n, uint32_t len, int32_t* out, Scratch& s) {
+void score_seqs(const Scoring& sc, const uint8_t* query, int qlen, const uint8_t* const* seqs,
+                const uint32_t* lens, uint32_t n, int32_t* out, Scratch& s) {
     if (n == 0) return;
-    if (qlen <= 0 || len == 0) {
+    if (qlen <= 0) {
         std::fill(out, out + n, 0);
         return;
     }
     Tables tb;
     make_tables(sc, query, qlen, tb);
     const int oe = sc.gap_open + sc.gap_extend, ge = sc.gap_extend;
-    // len is a multiple of 4 (the chunk format pads every sequence to one)
-    const uint32_t rows = (len + kRows - 1) / kRows * kRows;
-    std::vector<uint8_t> padded;
-    const uint8_t* src = seqs;
-    if (rows != len) {
-        padded.assign((size_t)n * rows, 23);
-        for (uint32_t k = 0; k < n; ++k)
-            std::memcpy(padded.data() + (size_t)k * rows, seqs + (size_t)k * len, len);
-        src = padded.data();
-    }
-    std::vector<uint32_t> all(n);
-    for (uint32_t k = 0; k < n; ++k) all[k] = k;
+    s.all.resize(n);
+    for (uint32_t k = 0; k < n; ++k) s.all[k] = k;
     s.ovf.clear();
     if (oe <= 127) {
         // int8: exact while no add saturated, i.e. while the score stays below
         // 256 - max(matrix)
-        std::vector<int32_t> s8(n);
-        run_groups<I8>(tb, query, qlen, src, rows, all.data(), n, oe, ge, s8.data(), s);
+        s.s8.resize(n);
+        run_groups<I8>(tb, query, qlen, seqs, lens, s.all.data(), n, oe, ge, s.s8.data(), s);
         for (uint32_t k = 0; k < n; ++k) {
-            if (s8[k] >= I8::kTop + 1 - tb.max_score) s.ovf.push_back(k);
-            else out[k] = s8[k];
+            if (s.s8[k] >= I8::kTop + 1 - tb.max_score) s.ovf.push_back(k);
+            else out[k] = s.s8[k];
         }
     } else {
-        s.ovf = all;
+        s.ovf = s.all;
     }
     if (s.ovf.empty()) return;
     s.ovf_scores.resize(s.ovf.size());
-    run_groups<I16>(tb, query, qlen, src, rows, s.ovf.data(), s.ovf.size(), oe, ge,
+    run_groups<I16>(tb, query, qlen, seqs, lens, s.ovf.data(), s.ovf.size(), oe, ge,
                     s.ovf_scores.data(), s);
     for (size_t i = 0; i < s.ovf.size(); ++i) {
         const int v = s.ovf_scores[i];
         out[s.ovf[i]] = v >= I16::kTop + 1 - tb.max_score ? -1 : v;
     }
+}
+
+void score_batch(const Scoring& sc, const uint8_t* query, int qlen, const uint8_t* seqs,
+                 uint32_t n, uint32_t len, int32_t* out, Scratch& s) {
+    std::vector<const uint8_t*> ptrs(n);
+    std::vector<uint32_t> lens(n, len);
+    for (uint32_t k = 0; k < n; ++k) ptrs[k] = seqs + (size_t)k * len;
+    score_seqs(sc, query, qlen, ptrs.data(), lens.data(), n, out, s);
 }
 
 }  // namespace cpu

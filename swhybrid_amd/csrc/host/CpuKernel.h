@@ -27,13 +27,20 @@ struct Scratch {
     std::vector<uint8_t> inter;     // interleaved residues of one lane group
     std::vector<uint8_t> colstate;  // H and E per query column
     std::vector<uint8_t> prof;      // profile of 4 rows
-    std::vector<uint32_t> ovf;      // overflowed sequences of a batch
+    std::vector<uint32_t> ovf;      // overflowed sequences of a call
     std::vector<int32_t> ovf_scores;
+    std::vector<int32_t> s8;
+    std::vector<uint32_t> all;
 };
 
-// Scores the n sequences of padded length len stored back to back at seqs
-// (codes; above 23 treated as 23).  out[k] = exact score, or -1 where the
-// int16 pass may have saturated.  query codes must be <= 23.
+// Scores n sequences: sequence k has lens[k] codes at seqs[k] (above 23
+// treated as 23).  Lane groups are formed in the given order, each run to its
+// longest member, so similar lengths should be adjacent.  out[k] = exact
+// score, or -1 where the int16 pass may have saturated.  Query codes <= 23.
+void score_seqs(const Scoring& sc, const uint8_t* query, int qlen, const uint8_t* const* seqs,
+                const uint32_t* lens, uint32_t n, int32_t* out, Scratch& scratch);
+
+// The same for n sequences of length len stored back to back at seqs.
 void score_batch(const Scoring& sc, const uint8_t* query, int qlen, const uint8_t* seqs,
                  uint32_t n, uint32_t len, int32_t* out, Scratch& scratch);
 

@@ -1,6 +1,7 @@
 #include "DataPool.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cstring>
 #include <numeric>
 
@@ -49,7 +50,8 @@ size_t DataPool::maxInfoPerChunk(size_t bufferSize) const {
 void DataPool::rewind() {
     std::lock_guard<std::mutex> g(mu_);
     next_ = 0;
-    handedOut_ = 0;
+    handedOut_ = handedOutCpu_ = handedOutDev_ = 0;
+    t0_ = 0;
 }
 
 bool DataPool::fillChunk(char* dst, batchInfo* info, size_t& infoSize, size_t& aaSize,
@@ -61,7 +63,22 @@ bool DataPool::fillChunk(char* dst, batchInfo* info, size_t& infoSize, size_t& a
         // claim a run of batches under the lock, copy it outside
         std::lock_guard<std::mutex> g(mu_);
         if (next_ >= batches_.size()) return false;
-        if (cpuFlag && totalPadded_ - handedOut_ <= cpuReserve_) return false;
+        const double t = std::chrono::duration<double>(
+                             std::chrono::steady_clock::now().time_since_epoch()).count();
+        if (t0_ == 0) t0_ = t;
+        if (cpuFlag) {
+            const size_t remaining = totalPadded_ - handedOut_;
+            if (remaining <= cpuReserve_) return false;
+            // Take a CPU chunk only if it should finish before the device
+            // workers, at the rates both kinds have drawn chunks so far, finish
+            // what would be left: a slow last CPU chunk lengthens the search.
+            const double el = t - t0_;
+            if (el > 0 && handedOutDev_ > 0 && handedOutCpu_ > 0) {
+                const double dev_rate = handedOutDev_ / el, cpu_rate = handedOutCpu_ / el;
+                const double chunk = (double)std::min(bufferSize, remaining);
+                if (chunk / cpu_rate > (remaining - chunk) / dev_rate) return false;
+            }
+        }
         b0 = next_;
         size_t used = 0;
         while (next_ < batches_.size()) {
@@ -74,6 +91,7 @@ bool DataPool::fillChunk(char* dst, batchInfo* info, size_t& infoSize, size_t& a
         }
         b1 = next_;
         handedOut_ += used;
+        (cpuFlag ? handedOutCpu_ : handedOutDev_) += used;
     }
     baseIndex = batches_[b0].first;
     size_t used = 0;

@@ -27,7 +27,8 @@ class DataPool {
     // Copies the next batches into dst while they fit in bufferSize bytes
     // (codes as given: consumers clamp codes above 23 when they pack).
     // Returns false (infoSize = 0) when the pool is exhausted, or, for a CPU
-    // worker (cpuFlag), once no more than the CPU reserve is left.
+    // worker (cpuFlag), once no more than the CPU reserve is left or once its
+    // chunk would outlast the device workers' share of the rest.
     bool fillChunk(char* dst, batchInfo* seqBatchInfo, size_t& infoSize, size_t& aaSize,
                    size_t bufferSize, bool cpuFlag, size_t& baseIndex);
 
@@ -61,6 +62,8 @@ class DataPool {
     std::vector<uint8_t> query_;
     size_t next_ = 0;
     size_t handedOut_ = 0;
+    size_t handedOutCpu_ = 0, handedOutDev_ = 0;   // per kind of worker
+    double t0_ = 0;                                // first fill, seconds
     size_t cpuReserve_ = 0;
     size_t totalPadded_ = 0;
     size_t maxBatchBytes_ = 0;

@@ -17,7 +17,21 @@ HIPWorker::HIPWorker(DataPool* loader, Param* params, RecalcWorker* recalcWorker
     bufferSize = chunkBytes;
 }
 
-HIPWorker::~HIPWorker() { free(); }
+HIPWorker::~HIPWorker() {
+    free();
+    freeChunk(dbSeqBuf);
+    dbSeqBuf = nullptr;
+}
+
+char* HIPWorker::allocChunk(size_t bytes) {
+    void* p = nullptr;
+    if (swh_pinned_alloc(bytes, &p) != SWH_OK) throw swh_fail("swh_pinned_alloc");
+    return static_cast<char*>(p);
+}
+
+void HIPWorker::freeChunk(char* p) {
+    if (p) (void)swh_pinned_free(p);
+}
 
 void HIPWorker::setQuery(const char* q, size_t qlen) {
     Worker::setQuery(q, qlen);

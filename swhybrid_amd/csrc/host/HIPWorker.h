@@ -4,9 +4,10 @@
 // src/CUDAWorker.cu:29-246) behind the same Worker interface, and reaches the
 // gfx950 kernels only through the C ABI of libswh.so (include/swh.h):
 //   alloc      swh_ctx_create on the worker's device
-//   packDB     the DataPool chunk becomes a packed, HBM-resident database
-//              (swh_db_create), on the aux thread while the previous chunk is
-//              searched
+//   packDB     the DataPool chunk (filled into page-locked memory) becomes a
+//              packed, HBM-resident database (swh_db_create: one DMA upload,
+//              interleaved on the device), on the aux thread while the
+//              previous chunk is searched
 //   asyncCopy  nothing left to copy: packDB uploaded the chunk
 //   search     swh_search -> final exact scores in compEntry->getScores();
 //              overflowed lanes were rescored on the device, so nothing goes
@@ -34,6 +35,8 @@ class HIPWorker : public Worker {
     void asyncCopy() override {}
     void search() override;
     void packDB() override;
+    char* allocChunk(size_t bytes) override;
+    void freeChunk(char* p) override;
     void showPerformance() override;
     void who() override;
 

@@ -19,9 +19,13 @@ Worker::Worker(DataPool* loader, Param* params, RecalcWorker* recalcWorker, int 
 }
 
 Worker::~Worker() {
-    std::free(dbSeqBuf);
+    std::free(dbSeqBuf);   // subclasses with their own allocChunk cleared it
     delete[] info;
 }
+
+char* Worker::allocChunk(size_t bytes) { return (char*)std::malloc(bytes); }
+
+void Worker::freeChunk(char* p) { std::free(p); }
 
 void Worker::setQuery(const char* q, size_t qlen) {
     query.assign(q, q + qlen);
@@ -31,7 +35,7 @@ void Worker::setQuery(const char* q, size_t qlen) {
 bool Worker::fill() {
     if (!dbSeqBuf) {
         bufferSize = std::max(bufferSize, loader_->maxBatchBytes());
-        dbSeqBuf = (char*)std::malloc(bufferSize + 64);
+        dbSeqBuf = allocChunk(bufferSize + 64);
         infoCapacity = loader_->maxInfoPerChunk(bufferSize) + 1;
         info = new batchInfo[infoCapacity];
         if (!dbSeqBuf) throw "worker: chunk buffer allocation failed\n";

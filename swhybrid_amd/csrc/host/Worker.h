@@ -81,6 +81,10 @@ class Worker {
   protected:
     virtual void who() {}
     virtual void packDB();
+    // the chunk buffer the DataPool fills (HIPWorker: page-locked memory);
+    // a subclass that overrides these frees dbSeqBuf in its own destructor
+    virtual char* allocChunk(size_t bytes);
+    virtual void freeChunk(char* p);
     bool fill();
     void auxWork();
     void swapEntries() {

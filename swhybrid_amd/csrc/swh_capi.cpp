@@ -33,6 +33,22 @@ int swh_device_count(int* count) {
     return SWH_OK;
 }
 
+int swh_pinned_alloc(size_t bytes, void** out) {
+    if (!out) return fail(SWH_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    hipError_t e = hipHostMalloc(out, std::max<size_t>(bytes, 1), hipHostMallocDefault);
+    if (e != hipSuccess) {
+        *out = nullptr;
+        return fail(SWH_ERR_NOMEM, "hipHostMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    }
+    return SWH_OK;
+}
+
+int swh_pinned_free(void* p) {
+    if (p) HIP_TRY(hipHostFree(p));
+    return SWH_OK;
+}
+
 int swh_ctx_create(int device, const int32_t* matrix, int gap_open, int gap_extend,
                    swh_ctx** out) {
     if (!out) return fail(SWH_ERR_ARG, "out is NULL");
@@ -52,7 +68,8 @@ int swh_ctx_create(int device, const int32_t* matrix, int gap_open, int gap_exte
         return fail(SWH_ERR_HIP, "hipGetDeviceProperties failed");
     }
     c->num_cus = prop.multiProcessorCount;
-    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->upload, hipStreamNonBlocking) != hipSuccess) {
         swh_ctx_destroy(c);
         return fail(SWH_ERR_HIP, "hipStreamCreate failed");
     }
@@ -74,6 +91,7 @@ int swh_ctx_destroy(swh_ctx* c) {
     if (!c) return SWH_OK;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->upload) (void)hipStreamSynchronize(c->upload);
     c->graph.release();
     c->border.release();
     c->counters.release();
@@ -88,6 +106,7 @@ int swh_ctx_destroy(swh_ctx* c) {
     for (auto& e : c->ev)
         if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
+    if (c->upload) (void)hipStreamDestroy(c->upload);
     delete c;
     return SWH_OK;
 }
@@ -172,59 +191,74 @@ int swh_db_create(swh_ctx* c, const uint8_t* residues, const uint64_t* offsets,
         db->blk_padded[b + 1] = db->blk_padded[b] + (uint64_t)bd.rows * n;
     }
     const uint64_t nslots = lane_first + (uint64_t)nl * kLaneSeqs + 2;
-    std::vector<uint32_t> words(std::max<uint64_t>(woff, 1));
     std::vector<uint32_t> perm(nslots, kNoSeq);
     std::vector<uint32_t> slen(nslots, 0);
     std::vector<uint32_t> sblk(nslots, 0);
-    auto put_seq = [&](uint64_t s, int b, uint32_t* w, uint32_t rows, size_t stride,
-                       size_t col) {
-        const uint8_t* src = nullptr;
-        uint32_t n = 0;
-        sblk[s] = (uint32_t)b;
-        if (s < nseq) {
-            uint32_t id = order[s];
-            perm[s] = id;
-            src = residues + offsets[id];
-            n = lengths[id];
-            slen[s] = n;
-        }
-        for (uint32_t k = 0; k < rows / 4; ++k) {
-            uint32_t v = 0;
-            for (int r = 0; r < 4; ++r) {
-                uint32_t i = 4 * k + r;
-                uint32_t code = i < n ? src[i] : 23u;
-                if (code > 23) code = 23;
-                v |= code << (8 * r);
-            }
-            w[(size_t)k * stride + col] = v;
-        }
-    };
     parallel_for(db->n_blocks, [&](int64_t b0, int64_t b1) {
         for (int64_t b = b0; b < b1; ++b) {
             const BlockDesc& bd = db->blocks[b];
-            uint32_t* w = words.data() + bd.word_off;
-            if (b < np) {
-                for (int half = 0; half < 2; ++half)
-                    put_seq(bd.first + half, (int)b, w, bd.rows, 2, half);
-            } else {
-                for (int half = 0; half < 2; ++half)
-                    for (int lane = 0; lane < kWave; ++lane)
-                        put_seq(bd.first + (uint64_t)half * kWave + lane, (int)b, w, bd.rows,
-                                (size_t)kWave * 2, (size_t)lane * 2 + half);
+            const uint64_t n = b < np ? 2 : kLaneSeqs;
+            for (uint64_t s = bd.first; s < bd.first + n; ++s) {
+                sblk[s] = (uint32_t)b;
+                if (s < nseq) {
+                    perm[s] = order[s];
+                    slen[s] = lengths[order[s]];
+                }
             }
         }
     });
+    // Residues go to the device as they are (the span the offsets cover, or
+    // a compact copy when that span is mostly gaps) and are interleaved there
+    // by k_pack: the host only plans the layout.
+    uint64_t lo = UINT64_MAX, hi = 0;
+    for (uint64_t i = 0; i < nseq; ++i)
+        if (lengths[i] > 0) {
+            lo = std::min(lo, offsets[i]);
+            hi = std::max(hi, offsets[i] + lengths[i]);
+        }
+    if (lo > hi) lo = hi = 0;
+    std::vector<uint64_t> soff(std::max<uint64_t>(nseq, 1), 0);
+    std::vector<uint8_t> compact;
+    const uint8_t* stage = residues + lo;
+    uint64_t stage_bytes = hi - lo;
+    if (stage_bytes > 2 * total + (uint64_t(64) << 20)) {
+        compact.resize(total);
+        uint64_t at = 0;
+        for (uint64_t i = 0; i < nseq; ++i) {
+            soff[i] = at;
+            at += lengths[i];
+        }
+        parallel_for((int64_t)nseq, [&](int64_t i0, int64_t i1) {
+            for (int64_t i = i0; i < i1; ++i)
+                std::memcpy(compact.data() + soff[i], residues + offsets[i], lengths[i]);
+        });
+        stage = compact.data();
+        stage_bytes = total;
+    } else {
+        for (uint64_t i = 0; i < nseq; ++i) soff[i] = lengths[i] > 0 ? offsets[i] - lo : 0;
+    }
     auto up = [&](DevBuf& d, const void* h, size_t bytes) -> hipError_t {
         hipError_t e = d.reserve(std::max<size_t>(bytes, 16));
-        if (e != hipSuccess) return e;
+        if (e != hipSuccess || bytes == 0) return e;
         return hipMemcpy(d.p, h, bytes, hipMemcpyHostToDevice);
     };
-    hipError_t e = up(db->words, words.data(), words.size() * 4);
+    DevBuf raw, d_soff;
+    hipError_t e = db->words.reserve(std::max<uint64_t>(woff, 1) * 4);
     if (e == hipSuccess && db->n_blocks > 0)
         e = up(db->d_blocks, db->blocks.data(), db->n_blocks * sizeof(BlockDesc));
     if (e == hipSuccess) e = up(db->perm, perm.data(), perm.size() * 4);
     if (e == hipSuccess) e = up(db->sorted_len, slen.data(), slen.size() * 4);
     if (e == hipSuccess) e = up(db->sorted_blk, sblk.data(), sblk.size() * 4);
+    if (e == hipSuccess) e = up(raw, stage, stage_bytes);
+    if (e == hipSuccess) e = up(d_soff, soff.data(), soff.size() * 8);
+    if (e == hipSuccess && db->n_blocks > 0)
+        e = launch_pack((const BlockDesc*)db->d_blocks.p, db->n_blocks, np,
+                        (const uint32_t*)db->perm.p, (const uint32_t*)db->sorted_len.p,
+                        (const uint8_t*)raw.p, (const uint64_t*)d_soff.p,
+                        (uint32_t*)db->words.p, c->upload);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->upload);
+    raw.release();
+    d_soff.release();
     if (e != hipSuccess) {
         swh_db_destroy(db);
         return fail(e == hipErrorOutOfMemory ? SWH_ERR_NOMEM : SWH_ERR_HIP,

@@ -145,7 +145,8 @@ struct GraphCache {
 struct swh_ctx {
     int device = 0;
     int num_cus = 0;
-    hipStream_t stream = nullptr;     // every launch and copy of the context
+    hipStream_t stream = nullptr;     // every launch and copy of a search
+    hipStream_t upload = nullptr;     // database packing (may overlap a search)
     int32_t matrix[1024];
     int gap_open = 10, gap_extend = 2, max_score = 0;
     DevBuf border, counters, prof, scores, ovf, desc;

@@ -88,6 +88,11 @@ struct RescoreArgs {
 
 // Launchers (swh_kernels.hip).
 hipError_t launch_reset(uint32_t* p, int n, hipStream_t s);
+// Interleaves raw residues (sequence id's codes at raw + soff[id]) into the
+// block layout; one workgroup per block.
+hipError_t launch_pack(const BlockDesc* blocks, int n_blocks, int n_pairs, const uint32_t* perm,
+                       const uint32_t* sorted_len, const uint8_t* raw, const uint64_t* soff,
+                       uint32_t* words, hipStream_t s);
 // pair: the lane scan reads a per-strip residue-pair table (widths 16, 24).
 hipError_t launch_scan(int lane_width, int group_width, bool pair, const ScanArgs& a, int grid,
                        size_t lds_bytes, hipStream_t s);

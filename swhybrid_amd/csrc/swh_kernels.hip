@@ -645,6 +645,45 @@ __global__ __launch_bounds__(256) void k_rescore(RescoreArgs A) {
     }
 }
 
+// Database packing: block b's words from the residues of its sequences.  Lane
+// blocks: word ((row/4)*64 + lane)*2 + half holds rows 4k..4k+3 of sorted
+// sequence first + half*64 + lane; pair blocks: word (row/4)*2 + half.  Codes
+// above 23 and rows beyond a sequence's end become 23.  Byte work: stores are
+// coalesced, loads are 4-byte reads of 64 different sequences per wave.
+__global__ __launch_bounds__(256) void k_pack(const BlockDesc* __restrict__ blocks, int n_pairs,
+                                              const uint32_t* __restrict__ perm,
+                                              const uint32_t* __restrict__ slen,
+                                              const uint8_t* __restrict__ raw,
+                                              const uint64_t* __restrict__ soff,
+                                              uint32_t* __restrict__ words) {
+    const int b = blockIdx.x;
+    const BlockDesc bd = blocks[b];
+    const bool pair = b < n_pairs;
+    const uint32_t per4 = pair ? 2u : (uint32_t)kLaneSeqs;
+    const uint64_t nw = (uint64_t)(bd.rows / 4) * per4;
+    for (uint64_t w = threadIdx.x; w < nw; w += blockDim.x) {
+        const uint32_t row4 = (uint32_t)(w / per4);
+        const uint32_t r = (uint32_t)(w - (uint64_t)row4 * per4);
+        const uint32_t s = pair ? bd.first + r : bd.first + (r & 1u) * kWave + (r >> 1);
+        uint32_t v = 0x17171717u;
+        const uint32_t id = perm[s];
+        if (id != kNoSeq) {
+            const uint32_t len = slen[s];
+            const uint32_t p0 = row4 * 4;
+            if (p0 < len) {
+                const uint8_t* src = raw + soff[id] + p0;
+                v = 0;
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    uint32_t c = p0 + k < len ? src[k] : 23u;
+                    v |= min(c, 23u) << (8 * k);
+                }
+            }
+        }
+        words[bd.word_off + w] = v;
+    }
+}
+
 __global__ void k_reset(uint32_t* p, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0;
@@ -653,6 +692,14 @@ __global__ void k_reset(uint32_t* p, int n) {
 // ---------------------------------------------------------------------------
 hipError_t launch_reset(uint32_t* p, int n, hipStream_t s) {
     hipLaunchKernelGGL(k_reset, dim3((n + 255) / 256), dim3(256), 0, s, p, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack(const BlockDesc* blocks, int n_blocks, int n_pairs, const uint32_t* perm,
+                       const uint32_t* sorted_len, const uint8_t* raw, const uint64_t* soff,
+                       uint32_t* words, hipStream_t s) {
+    hipLaunchKernelGGL(k_pack, dim3(n_blocks), dim3(256), 0, s, blocks, n_pairs, perm,
+                       sorted_len, raw, soff, words);
     return hipGetLastError();
 }
 
