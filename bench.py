@@ -254,7 +254,7 @@ def run_serving(eng, d, qps_levels, n_requests):
     res = {}
     for qps in qps_levels:
         arrivals, qs = G.c4_requests(n_requests, qps)
-        srv = Server(eng, d, slices=4)
+        srv = Server(eng, d)
         ids = [None] * len(qs)
         t0 = time.perf_counter()
 

@@ -183,8 +183,8 @@ int swh_get_tuning(const swh_ctx* ctx, swh_tune_entry* entries, int cap, int* n)
  * the device memory their query profile, score vector and overflow list need
  * fits the budget; each round then runs ONE persistent scan launch over the
  * next database slice of every admitted request, so a request that arrives
- * while others are on the GPU joins them at the next slice boundary and their
- * work overlaps on the device.  Scores are exact and independent of which
+ * while others are on the GPU joins them at the next round and their work
+ * overlaps on the device.  Scores are exact and independent of which
  * requests share a round.  No reference counterpart (the reference runs one
  * query per process). */
 typedef struct swh_server swh_server;
@@ -192,7 +192,10 @@ typedef struct swh_server swh_server;
 typedef struct swh_server_opts {
     int max_inflight;          /* requests on the device at once (0 = 64) */
     uint64_t memory_budget;    /* device bytes for request buffers (0 = 4 GiB) */
-    int slices;                /* database slices per request (0 = 4) */
+    int slices;                /* database slices per request, each a round that new
+                                  requests can join (0 = 1: on MI355X every slice
+                                  adds a launch tail; one slice gave the lowest
+                                  p50 and p99 at a sustainable load) */
     int max_qlen;              /* longest admissible query (0 = 8192) */
     int max_k;                 /* largest top-k a request may ask for (0 = 100) */
     swh_search_opts search;    /* strip widths, overflow threshold */

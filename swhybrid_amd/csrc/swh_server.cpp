@@ -262,7 +262,7 @@ int swh_server_create(swh_ctx* c, swh_db* db, const swh_server_opts* opts, swh_s
     if (opts) o = *opts;
     if (!o.max_inflight) o.max_inflight = 64;
     if (!o.memory_budget) o.memory_budget = uint64_t(4) << 30;
-    if (!o.slices) o.slices = 4;
+    if (!o.slices) o.slices = 1;
     if (!o.max_qlen) o.max_qlen = 8192;
     if (!o.max_k) o.max_k = 100;
     if (o.max_inflight < 1 || o.slices < 1 || o.max_qlen < 1 || o.max_k < 1)
