@@ -138,9 +138,9 @@ void CPUWorker::search() {
 
 void CPUWorker::showPerformance() {
     const double t = getStopTime() - getStartTime();
-    std::printf("CPU[%d threads] computing time %lfs, AARes %zu, GCUPs %lf\n", threads_, t,
+    std::fprintf(stderr, "CPU[%d threads] computing time %lfs, AARes %zu, GCUPs %lf\n", threads_, t,
                 TotalAminoAcidResidue,
                 t > 0 ? TotalAminoAcidResidue * (double)queryLen / t / 1e9 : 0.0);
 }
 
-void CPUWorker::who() { std::printf("CPU\n"); }
+void CPUWorker::who() { std::fprintf(stderr, "CPU\n"); }

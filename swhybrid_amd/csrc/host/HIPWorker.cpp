@@ -98,8 +98,8 @@ void HIPWorker::search() {
 
 void HIPWorker::showPerformance() {
     const double t = getStopTime() - getStartTime();
-    std::printf("HIP[%d] computing time %lfs, AARes %zu, GCUPs %lf\n", deviceId_, t,
+    std::fprintf(stderr, "HIP[%d] computing time %lfs, AARes %zu, GCUPs %lf\n", deviceId_, t,
                 TotalAminoAcidResidue, t > 0 ? TotalAminoAcidResidue * (double)queryLen / t / 1e9 : 0.0);
 }
 
-void HIPWorker::who() { std::printf("HIP\n"); }
+void HIPWorker::who() { std::fprintf(stderr, "HIP\n"); }

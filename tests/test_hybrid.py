@@ -112,3 +112,13 @@ def test_int16_overflow_rescored_exactly():
     db = G.SeqDB.from_list(seqs)
     r = check(db, q, k=3, hip_workers=0, cpu_threads=2)
     assert r["report"]["recalc"] == 2 and r["top_score"][0] == 66000
+
+
+def test_pool_prints_nothing_on_stdout(capfd):
+    """Library calls keep stdout clean (bench.py prints exactly one JSON line
+    there); the workers' performance lines go to stderr."""
+    db = mixed_db(77, n=300)
+    q = G.queries([50], 78)[0]
+    hybrid_search(db, q, k=3, hip_workers=0, cpu_threads=2)
+    out, err = capfd.readouterr()
+    assert out == "" and "CPU[" in err
