@@ -4,6 +4,7 @@
 #include <chrono>
 #include <cstring>
 #include <numeric>
+#include <thread>
 
 static const unsigned kBatchWidth = 128;   // src/Defs.h:15
 
@@ -94,19 +95,42 @@ bool DataPool::fillChunk(char* dst, batchInfo* info, size_t& infoSize, size_t& a
         (cpuFlag ? handedOutCpu_ : handedOutDev_) += used;
     }
     baseIndex = batches_[b0].first;
-    size_t used = 0;
-    for (size_t bi = b0; bi < b1; ++bi) {
-        const Batch& b = batches_[bi];
-        char* out = dst + used;
-        for (unsigned k = 0; k < b.num; ++k) {
-            const uint32_t id = order_[b.first + k];
-            const uint32_t n = std::min<uint32_t>(lengths_[id], b.len);
-            std::memcpy(out + (size_t)k * b.len, residues_ + offsets_[id], n);
-            std::memset(out + (size_t)k * b.len + n, kPadCode, b.len - n);
+    // copy the claimed batches; a large chunk is split over a few threads
+    std::vector<size_t> at(b1 - b0 + 1, 0);
+    for (size_t bi = b0; bi < b1; ++bi)
+        at[bi - b0 + 1] = at[bi - b0] + (size_t)batches_[bi].num * batches_[bi].len;
+    auto copy = [&](size_t lo, size_t hi) {
+        for (size_t bi = lo; bi < hi; ++bi) {
+            const Batch& b = batches_[bi];
+            char* out = dst + at[bi - b0];
+            for (unsigned k = 0; k < b.num; ++k) {
+                const uint32_t id = order_[b.first + k];
+                const uint32_t n = std::min<uint32_t>(lengths_[id], b.len);
+                std::memcpy(out + (size_t)k * b.len, residues_ + offsets_[id], n);
+                std::memset(out + (size_t)k * b.len + n, kPadCode, b.len - n);
+            }
         }
-        info[infoSize++] = batchInfo(b.num, b.len);
-        used += (size_t)b.num * b.len;
+    };
+    const size_t used = at.back();
+    const unsigned nt = used >= (size_t(32) << 20) ? 8u : 1u;
+    if (nt == 1 || b1 - b0 < 2 * nt) {
+        copy(b0, b1);
+    } else {
+        // split by bytes: thread t takes the batches starting in its share
+        std::vector<size_t> cut(nt + 1, b0);
+        cut[nt] = b1;
+        size_t bi = b0;
+        for (unsigned t = 1; t < nt; ++t) {
+            const size_t target = used * t / nt;
+            while (bi < b1 && at[bi - b0] < target) ++bi;
+            cut[t] = bi;
+        }
+        std::vector<std::thread> team;
+        for (unsigned t = 1; t < nt; ++t) team.emplace_back(copy, cut[t], cut[t + 1]);
+        copy(cut[0], cut[1]);
+        for (auto& th : team) th.join();
     }
+    for (size_t bi = b0; bi < b1; ++bi) info[infoSize++] = batchInfo(batches_[bi].num, batches_[bi].len);
     aaSize = used;
     return true;
 }
