@@ -10,8 +10,9 @@ Prints ONE JSON line (metric GCUPS, higher is better).  Extra keys:
   roofline      the dominant scan kernel's integer VALU rate against the
                 MI355X packed-int16 peak (HIP events on the search stream)
   cpu_baseline  the reference's own AVX2 kernels (oracle/_ref, built from the
-                reference sources) on a 1/16 sample of the database, timed on
-                this host's cores; the oracle's C restatement if _ref is absent
+                reference sources) on every 3rd sequence of the database (about
+                10 s of work), timed on this host's cores; the oracle's C
+                restatement if _ref is absent
   per_query     GCUPS and kernel time for each query length
 """
 from __future__ import annotations
@@ -37,7 +38,7 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(db, queries, matrix, go, ge, stride=16):
+def cpu_baseline(db, queries, matrix, go, ge, stride=3):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     from swhybrid_amd.datagen import SeqDB
@@ -380,6 +381,11 @@ def main():
         "kernel_gcups": round(cells / (kernel_ms / 1e3) / 1e9, 1),
         "avg_launch_ms": round(kernel_ms / (args.steps * len(queries)), 2),
     }
+    if roofline["traffic"]:
+        # the HBM side of the same launches: measured bytes / launch time
+        gbs = roofline["traffic"] / (roofline["avg_launch_ms"] / 1e3) / 1e9
+        roofline["hbm"] = {"achieved": round(gbs, 1), "peak": 8000.0, "unit": "GB/s",
+                           "frac": round(gbs / 8000.0, 4)}
     pq_out = {str(k): {"gcups": round(v["cells"] / (v["total_ms"] / 1e3) / 1e9, 1),
                        "kernel_gcups": round(v["cells"] / (v["kernel_ms"] / 1e3) / 1e9, 1),
                        "kernel_ms": round(v["kernel_ms"] / args.steps, 2)}
