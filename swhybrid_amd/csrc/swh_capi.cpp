@@ -785,16 +785,27 @@ int swh_topk(const int32_t* scores, uint64_t n, int k, int32_t* out_score, int64
     };
     std::vector<std::pair<int32_t, int64_t>> heap;
     heap.reserve(k + 1);
-    for (uint64_t i = 0; i < n && k > 0; ++i) {
-        const int32_t s = scores[i];
-        if (s <= 0) continue;
-        if ((int)heap.size() < k) {
-            heap.emplace_back(s, (int64_t)i);
-            std::push_heap(heap.begin(), heap.end(), worse);
-        } else if (s > heap.front().first) {
-            std::pop_heap(heap.begin(), heap.end(), worse);
-            heap.back() = {s, (int64_t)i};
-            std::push_heap(heap.begin(), heap.end(), worse);
+    // once the heap is full a score must beat its weakest entry; blocks of 64
+    // whose maximum cannot are skipped (a vectorisable max per block)
+    int32_t floor = 0;   // scores <= floor cannot enter
+    constexpr uint64_t kBlock = 64;
+    for (uint64_t b = 0; b < n && k > 0; b += kBlock) {
+        const uint64_t e = std::min(n, b + kBlock);
+        int32_t m = scores[b];
+        for (uint64_t i = b + 1; i < e; ++i) m = std::max(m, scores[i]);
+        if (m <= floor) continue;
+        for (uint64_t i = b; i < e; ++i) {
+            const int32_t s = scores[i];
+            if (s <= floor) continue;
+            if ((int)heap.size() < k) {
+                heap.emplace_back(s, (int64_t)i);
+                std::push_heap(heap.begin(), heap.end(), worse);
+            } else {
+                std::pop_heap(heap.begin(), heap.end(), worse);
+                heap.back() = {s, (int64_t)i};
+                std::push_heap(heap.begin(), heap.end(), worse);
+            }
+            if ((int)heap.size() == k) floor = heap.front().first;
         }
     }
     std::sort(heap.begin(), heap.end(), worse);

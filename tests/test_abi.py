@@ -48,8 +48,11 @@ def test_bad_arguments_are_rejected():
 
 def test_host_topk_matches_resultlist():
     rng = np.random.Generator(np.random.PCG64(1))
-    for n, k, hi in [(10000, 20, 60), (50, 10, 3), (7, 10, 5), (0, 4, 1)]:
+    cases = [(10000, 20, 60), (50, 10, 3), (7, 10, 5), (0, 4, 1), (100_000, 100, 70000)]
+    cases += [(int(rng.integers(0, 3000)), int(rng.integers(0, 40)), int(rng.integers(1, 60)))
+              for _ in range(100)]                       # block skipping across ties
+    for n, k, hi in cases:
         s = rng.integers(-5, hi, size=n).astype(np.int32)
         a = topk(s, k)
         b = O.topk(s, k)
-        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]), (n, k, hi)
