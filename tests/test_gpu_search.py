@@ -335,3 +335,36 @@ def test_tuning_table_validation():
         e.set_tuning([good, {**good, "qlen_max": 50}])     # not ascending
     e.set_tuning([good])
     e.close()
+
+
+# --- extremes ------------------------------------------------------------------------
+
+def test_very_long_query(eng):
+    db = small_db(120, 901, mean=300, hi=1500)
+    q = G.queries([9000], 902)[0]
+    for w in (16, 24, 32):
+        check(eng, db, q, strip_width=w)
+
+
+def test_very_long_database_sequences(eng):
+    rng = np.random.Generator(np.random.PCG64(903))
+    q = G.queries([300], 904)[0]
+    seqs = [G.random_residues(rng, 60_000), G.random_residues(rng, 45_001),
+            G.mutate(rng, np.concatenate([G.random_residues(rng, 30_000), q]))]
+    seqs += [G.random_residues(rng, int(n)) for n in rng.integers(1, 500, size=40)]
+    db = G.SeqDB.from_list(seqs)
+    check_db(eng, db, q, min_len=2048)
+    check_db(eng, db, q, min_len=2048, group_lanes=4)
+    check_db(eng, db, q, min_len=-1)                  # the same sequences on single lanes
+
+
+def test_many_queries_one_launch(eng):
+    rng = np.random.Generator(np.random.PCG64(905))
+    db = small_db(200, 906, mean=150, hi=600)
+    qs = [G.random_residues(rng, int(m)) for m in rng.integers(1, 200, size=300)]
+    d = eng.load_db(db)
+    got, st = eng.search_batch(d, qs)
+    for i in range(0, len(qs), 7):
+        assert np.array_equal(got[i], O.search_i32(qs[i], db, eng.matrix, 10, 2)), f"query {i}"
+    assert st["cells"] == sum(db.total * len(q) for q in qs)
+    d.close()
