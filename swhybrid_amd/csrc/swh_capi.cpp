@@ -434,7 +434,8 @@ Shape choose_shape(const swh_ctx* c, const int* qlens, int nq, const swh_search_
 // the int32 rescore of their overflow lists.  counters needs 16 + nt words;
 // counters[16 + i] ends up holding task i's overflow count.
 int enqueue_tasks(swh_ctx* c, swh_db* db, const Task* t, int nt, const Shape& sh,
-                         const swh_search_opts& o, uint32_t* counters, swh_stats* st) {
+                  const swh_search_opts& o, uint32_t* counters, const LaunchRes& lr,
+                  swh_stats* st) {
     const int exact_thr = 65535 - c->max_score;
     const int thr = (o.overflow_threshold > 0) ? std::min(o.overflow_threshold, exact_thr)
                                                : exact_thr;
@@ -526,10 +527,10 @@ int enqueue_tasks(swh_ctx* c, swh_db* db, const Task* t, int nt, const Shape& sh
     std::make_heap(heap.begin(), heap.end());
     const size_t desc_bytes = 3 * (size_t)nt * sizeof(QueryDesc);
     const size_t total_bytes = desc_bytes + (size_t)n_items * sizeof(uint2);
-    HIP_TRY(c->h_desc.reserve(total_bytes));
-    HIP_TRY(c->desc.reserve(total_bytes));
-    std::memcpy(c->h_desc.p, hd.data(), desc_bytes);
-    uint2* hitems = (uint2*)((char*)c->h_desc.p + desc_bytes);
+    HIP_TRY(lr.h_desc->reserve(total_bytes));
+    HIP_TRY(lr.desc->reserve(total_bytes));
+    std::memcpy(lr.h_desc->p, hd.data(), desc_bytes);
+    uint2* hitems = (uint2*)((char*)lr.h_desc->p + desc_bytes);
     for (int n = 0; !heap.empty(); ++n) {
         std::pop_heap(heap.begin(), heap.end());
         Head& h = heap.back();
@@ -543,14 +544,14 @@ int enqueue_tasks(swh_ctx* c, swh_db* db, const Task* t, int nt, const Shape& sh
             heap.pop_back();
         }
     }
-    c->desc_used = total_bytes;
-    HIP_TRY(hipMemcpyAsync(c->desc.p, c->h_desc.p, total_bytes, hipMemcpyHostToDevice,
-                           c->stream));
-    const QueryDesc* d_lane = (const QueryDesc*)c->desc.p;
+    *lr.desc_used = total_bytes;
+    HIP_TRY(hipMemcpyAsync(lr.desc->p, lr.h_desc->p, total_bytes, hipMemcpyHostToDevice,
+                           lr.stream));
+    const QueryDesc* d_lane = (const QueryDesc*)lr.desc->p;
     const QueryDesc* d_group = d_lane + nt;
     const QueryDesc* d_res = d_lane + 2 * nt;
-    const uint2* d_items = (const uint2*)((const char*)c->desc.p + desc_bytes);
-    HIP_TRY(launch_reset(counters, 16 + nt, c->stream));
+    const uint2* d_items = (const uint2*)((const char*)lr.desc->p + desc_bytes);
+    HIP_TRY(launch_reset(counters, 16 + nt, lr.stream));
 
     // scan: every item of every task, one launch
     const int ps_l = std::min(std::max(max_qpad_l, 8), kSegCols) + kProfPad;
@@ -574,7 +575,7 @@ int enqueue_tasks(swh_ctx* c, swh_db* db, const Task* t, int nt, const Shape& sh
     const uint64_t r_per_wg = rstride * sizeof(uint2) * kWavesPerWG;
     int rgrid = c->num_cus * rescore_max_wg_per_cu(lds_r);
     rgrid = (int)std::max<uint64_t>(1, std::min<uint64_t>(rgrid, (size_t(1) << 30) / r_per_wg));
-    HIP_TRY(c->border.reserve(
+    HIP_TRY(lr.border->reserve(
         std::max<uint64_t>({16, per_wg * (uint64_t)grid, r_per_wg * (uint64_t)rgrid})));
 
     ScanArgs a{};
@@ -585,7 +586,7 @@ int enqueue_tasks(swh_ctx* c, swh_db* db, const Task* t, int nt, const Shape& sh
     a.lane_q = d_lane;
     a.items = d_items;
     a.work_counter = counters + 0;
-    a.border = (uint2*)c->border.p;
+    a.border = (uint2*)lr.border->p;
     a.border_stride = stride;
     a.border_rows = db->max_rows_pair;
     a.n_queries = nt;
@@ -595,7 +596,7 @@ int enqueue_tasks(swh_ctx* c, swh_db* db, const Task* t, int nt, const Shape& sh
     a.gap_oe = gap_oe;
     a.gap_e = c->gap_extend;
     if (items > 0) {
-        HIP_TRY(launch_scan(sh.Wl, sh.Wg, sh.pair, a, grid, lds, c->stream));
+        HIP_TRY(launch_scan(sh.Wl, sh.Wg, sh.pair, a, grid, lds, lr.stream));
         st->n_launches += 1;
     }
     RescoreArgs r{};
@@ -606,14 +607,14 @@ int enqueue_tasks(swh_ctx* c, swh_db* db, const Task* t, int nt, const Shape& sh
     r.sorted_blk = (const uint32_t*)db->sorted_blk.p;
     r.queries = d_res;
     r.work_counter = counters + 2;
-    r.border = (uint2*)c->border.p;
+    r.border = (uint2*)lr.border->p;
     r.border_stride = rstride;
     r.n_queries = nt;
     r.n_lane_blocks_first = db->n_pairs;
     r.ps = ps_r;
     r.gap_oe = gap_oe;
     r.gap_e = c->gap_extend;
-    HIP_TRY(launch_rescore(r, rgrid, lds_r, c->stream));
+    HIP_TRY(launch_rescore(r, rgrid, lds_r, lr.stream));
     st->n_launches += 1;
     return SWH_OK;
 }
@@ -669,7 +670,7 @@ int run_queries(swh_ctx* c, swh_db* db, const uint8_t* const* qs, const int* qle
     uint32_t* ctr = (uint32_t*)c->counters.p;
     // the launch sequence after the profile upload: scan, rescore, copy-back
     auto enqueue_rest = [&](swh_stats* pst) -> int {
-        int r2 = enqueue_tasks(c, db, tasks.data(), nq, sh, o, ctr, pst);
+        int r2 = enqueue_tasks(c, db, tasks.data(), nq, sh, o, ctr, ctx_res(c), pst);
         if (r2) return r2;
         HIP_TRY(hipMemcpyAsync(c->h_ctr.p, ctr, (16 + (size_t)nq) * sizeof(uint32_t),
                                hipMemcpyDeviceToHost, c->stream));

@@ -208,8 +208,23 @@ uint64_t range_padded(const swh_db* db, int b0, int b1);
 int check_opts(const swh_search_opts& o);
 int check_query(const uint8_t* q, int qlen, int i);
 Shape choose_shape(const swh_ctx* c, const int* qlens, int nq, const swh_search_opts& o);
+
+// What one launch sequence owns: its stream, descriptor staging and strip
+// borders.  Two of them let two launch sequences be in flight at once.
+struct LaunchRes {
+    hipStream_t stream;
+    DevBuf* desc;
+    PinnedBuf* h_desc;
+    DevBuf* border;
+    size_t* desc_used;   // bytes of h_desc the last enqueue uploaded
+};
+inline LaunchRes ctx_res(swh_ctx* c) {
+    return LaunchRes{c->stream, &c->desc, &c->h_desc, &c->border, &c->desc_used};
+}
+
 int enqueue_tasks(swh_ctx* c, swh_db* db, const Task* t, int nt, const Shape& sh,
-                  const swh_search_opts& o, uint32_t* counters, swh_stats* st);
+                  const swh_search_opts& o, uint32_t* counters, const LaunchRes& lr,
+                  swh_stats* st);
 // Search every query over the whole database; scores_host gets nq rows of nseq.
 int run_queries(swh_ctx* c, swh_db* db, const uint8_t* const* qs, const int* qlens, int nq,
                 const swh_search_opts& o, int32_t* scores_host, swh_stats* stats);

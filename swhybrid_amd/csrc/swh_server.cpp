@@ -148,7 +148,7 @@ bool run_round(swh_server* s) {
     const Shape sh = choose_shape(c, qlens.data(), (int)qlens.size(), s->opts.search);
     swh_stats st{};
     int rc = enqueue_tasks(c, db, tasks.data(), (int)tasks.size(), sh, s->opts.search,
-                           (uint32_t*)s->counters.p, &st);
+                           (uint32_t*)s->counters.p, ctx_res(c), &st);
     if (rc != SWH_OK) {
         s->loop_error = g_err;
         return false;
