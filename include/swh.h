@@ -64,8 +64,9 @@ typedef struct swh_stats {
 typedef struct swh_search_opts {
     int strip_width;          /* query columns held in registers per lane:
                                  0 = auto, else 16, 24 or 32 */
-    int overflow_threshold;   /* 0 = exact default (65535 - max(matrix));
-                                 a lower value forces more int32 rescoring */
+    int overflow_threshold;   /* 0 = exact default (31744: the int16 scan is exact
+                                 below it); a lower value forces more int32
+                                 rescoring */
     int group_lanes;          /* lanes cooperating on one long sequence pair
                                  (1..64); 0 = auto */
     int group_width;          /* query columns per lane in group mode: 0 = auto,

@@ -436,7 +436,9 @@ Shape choose_shape(const swh_ctx* c, const int* qlens, int nq, const swh_search_
 int enqueue_tasks(swh_ctx* c, swh_db* db, const Task* t, int nt, const Shape& sh,
                   const swh_search_opts& o, uint32_t* counters, const LaunchRes& lr,
                   swh_stats* st) {
-    const int exact_thr = 65535 - c->max_score;
+    // exact below both the saturation of an add (65535 - max(matrix)) and the
+    // fp16 ordering of the three-way max
+    const int exact_thr = std::min(65535 - c->max_score, kScoreLimitF16);
     const int thr = (o.overflow_threshold > 0) ? std::min(o.overflow_threshold, exact_thr)
                                                : exact_thr;
     const int gap_oe = c->gap_open + c->gap_extend;

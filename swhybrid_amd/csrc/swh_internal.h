@@ -14,6 +14,10 @@ constexpr int kSegCols = 1344;       // query columns per LDS profile segment (m
                                      // 24 rows x (1344+8) int16 = 64,896 B of LDS)
 constexpr int kProfPad = 8;          // extra int16 per LDS row: rows start 16 B apart mod 128 B
 constexpr uint32_t kNoSeq = 0xffffffffu;
+// The int16 scan is exact for scores below this (its three-way max reads the
+// biased patterns as fp16 values, which order only below 0x7C00); sequences
+// that reach it are rescored in int32.
+constexpr int kScoreLimitF16 = 31744;
 
 // Per-block descriptor of the packed database.  Lane-mode blocks hold 128
 // sequences: lane l carries sorted sequences first+l (low half) and

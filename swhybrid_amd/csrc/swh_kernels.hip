@@ -10,7 +10,8 @@
 //
 // Scan kernel (int16): every lane carries TWO database sequences, one in each
 // 16-bit half of a VGPR, and all arithmetic is packed and saturating
-// (v_pk_add_i16 clamp / v_pk_sub_i16 clamp / v_pk_max_i16).  Values are biased
+// (v_pk_add_i16 clamp / v_pk_sub_i16 clamp / v_pk_max_i16, and for the
+// three-way maxima v_pk_minimum3_f16 on the same bit patterns, see vmax3).  Values are biased
 // so that 0x8000 encodes score 0: the saturating floor at -32768 is the
 // local-alignment zero floor, exactly as the reference's 0x80 / 0x8000 bias
 // (src/AVXKernel.cpp:443, :903).  A lane keeps a strip of W query columns in
@@ -21,9 +22,10 @@
 // lane's two residues with 16-byte LDS loads and interleaves them with one
 // v_perm_b32 per column.
 //
-// A sequence whose int16 score reaches the overflow threshold (default
-// 65535 - max(matrix), below which no add can have saturated) is appended to
-// an overflow list and rescored exactly by the int32 kernel, on the device.
+// A sequence whose int16 score reaches the overflow threshold (default 31744,
+// below which no add can have saturated and the three-way max orders) is
+// appended to an overflow list and rescored exactly by the int32 kernel, on
+// the device.
 #include "swh_internal.h"
 
 namespace swh {
@@ -37,6 +39,23 @@ __device__ __forceinline__ int adds(int a, int b) { return __builtin_elementwise
 __device__ __forceinline__ int subs(int a, int b) { return __builtin_elementwise_sub_sat(a, b); }
 __device__ __forceinline__ int vmax(int a, int b) { return a > b ? a : b; }
 __device__ __forceinline__ uint32_t bits(s2 v) { return __builtin_bit_cast(uint32_t, v); }
+
+// Three-way max of biased int16 scores in one instruction.  The bit pattern
+// 0x8000 + s of a score s is, read as an fp16, the negative number -f(s) with
+// f increasing in s while s < 0x7C00 (subnormals included: the kernels run
+// with fp16 denormals enabled), so the IEEE minimum of the patterns is the
+// pattern of the largest score: v_pk_minimum3_f16 does two packed max ops.
+// Patterns of scores from 31744 on are -inf or NaN and do not order; the scan
+// sends every sequence whose score reaches kScoreLimitF16 to the int32
+// rescore, and such a pattern, once formed, makes the score reach it.
+typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ s2 vmax3(s2 a, s2 b, s2 c) {
+    const h2 x = __builtin_bit_cast(h2, a), y = __builtin_bit_cast(h2, b),
+             z = __builtin_bit_cast(h2, c);
+    return __builtin_bit_cast(s2, __builtin_elementwise_minimum(
+                                      __builtin_elementwise_minimum(x, y), z));
+}
+__device__ __forceinline__ int vmax3(int a, int b, int c) { return vmax(vmax(a, b), c); }
 __device__ __forceinline__ s2 as_s2(uint32_t v) { return __builtin_bit_cast(s2, v); }
 
 constexpr uint32_t kZero16x2 = 0x80008000u;   // biased zero in both halves
@@ -69,7 +88,7 @@ __device__ __forceinline__ void row_cells(T (&h)[W], T (&e)[W], T (&hq)[W], cons
         e[c] = vmax(subs(e[c], R), hq[c]);         // E(i, c)
         T Xn = X;
         if (c + 1 < W) Xn = adds(h[c], P[c + 1]);  // diagonal term of column c+1
-        T hn = vmax(vmax(X, e[c]), f);             // H(i, c); zero floor by saturation
+        T hn = vmax3(X, e[c], f);                  // H(i, c); zero floor by saturation
         hq[c] = subs(hn, Q);
         h[c] = hn;
         hqp = hq[c];
@@ -78,17 +97,28 @@ __device__ __forceinline__ void row_cells(T (&h)[W], T (&e)[W], T (&hq)[W], cons
     fout = f;
 }
 
-// Maximum of a row as a balanced tree (W-1 independent-ish max ops, no state
-// carried between rows), folded into the running score once per row.
+// The running score folded with the maximum of a row: a tree of three-way
+// maxima over the W values and S (no state carried between rows).
 template <int W, typename T>
-__device__ __forceinline__ T row_max(const T (&h)[W]) {
-    T m[W];
+__device__ __forceinline__ T row_max_into(T S, const T (&h)[W]) {
+    T m[W + 1];
 #pragma unroll
     for (int c = 0; c < W; ++c) m[c] = h[c];
+    m[W] = S;
+    int n = W + 1;
 #pragma unroll
-    for (int step = 1; step < W; step <<= 1)
+    for (int level = 0; level < 8; ++level) {
+        if (n == 1) break;
+        int k = 0;
 #pragma unroll
-        for (int c = 0; c + step < W; c += 2 * step) m[c] = vmax(m[c], m[c + step]);
+        for (int i = 0; i < W + 1; i += 3) {
+            if (i >= n) break;
+            if (i + 2 < n) m[k++] = vmax3(m[i], m[i + 1], m[i + 2]);
+            else if (i + 1 < n) m[k++] = vmax(m[i], m[i + 1]);
+            else m[k++] = m[i];
+        }
+        n = k;
+    }
     return m[0];
 }
 
@@ -264,7 +294,7 @@ __device__ __forceinline__ void lane_strip(const uint2* __restrict__ wds, int qu
             const s2 fl = as_s2(bc[r].y);
             s2 fout;
             row_cells<W, s2>(h, e, hq, P, diag0, hl, fl, Q, R, fout);
-            S = vmax(S, row_max<W, s2>(h));
+            S = row_max_into<W, s2>(S, h);
             if (OUT) bo[r * kWave] = make_uint2(bits(h[W - 1]), bits(fout));
             diag0 = hl;
             __builtin_amdgcn_sched_barrier(0);
@@ -454,7 +484,7 @@ __device__ __forceinline__ void group_item(const ScanArgs& A, const QueryDesc& Q
                     pair_profile<W>(P, lds, A.ps_group, a, b, jj0);
                     s2 fout;
                     row_cells<W, s2>(h, e, hq, P, diag0, as_s2(hl_u), as_s2(fl_u), Q, R, fout);
-                    S = vmax(S, row_max<W, s2>(h));
+                    S = row_max_into<W, s2>(S, h);
                     ch = bits(h[W - 1]);
                     cf = bits(fout);
                     diag0 = as_s2(hl_u);
@@ -629,7 +659,7 @@ __global__ __launch_bounds__(256) void k_rescore(RescoreArgs A) {
                         int fout;
                         row_cells<W, int>(h, e, hq, P, diag0, (int)bin.x, (int)bin.y,
                                           A.gap_oe, A.gap_e, fout);
-                        S = vmax(S, row_max<W, int>(h));
+                        S = row_max_into<W, int>(S, h);
                         if (has_out)
                             bord[(size_t)row * kWave + lane] =
                                 make_uint2((uint32_t)h[W - 1], (uint32_t)fout);
