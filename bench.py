@@ -30,6 +30,8 @@ sys.path.insert(0, ROOT)
 
 # MI355X: 256 CUs x 2.4 GHz x 64 lanes x 2 (packed 16-bit) lane-ops per clock
 PEAK_INT16_TOPS = 256 * 2.4e9 * 64 * 2 / 1e12
+# VALU issue: one wave64 instruction per SIMD every 4 clocks = 64 lane-instructions per CU clock
+PEAK_VALU_TIPS = 256 * 2.4e9 * 64 / 1e12
 OPS_PER_CELL = 9   # 1 add, 3 sub, 5 max (the reference's ONE_CELL_UPDATE)
 TOP_K = 20
 
@@ -69,18 +71,18 @@ def cpu_baseline(db, queries, matrix, go, ge, stride=3):
     }
 
 
-def measured_traffic():
-    """HBM bytes per k_scan launch, averaged over the five query lengths of
-    this workload, from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs
-    (calibrated on a 1 GiB copy); the counters cannot be read from inside a
-    timed run."""
+def measured_counters():
+    """HBM bytes per k_scan launch and VALU instructions per cell pair,
+    averaged over the five query lengths of this workload, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE (calibrated on a 1 GiB copy) and
+    SQ_INSTS_VALU runs; the counters cannot be read from inside a timed run."""
     path = os.path.join(ROOT, "profiles", "k_scan_traffic.json")
     try:
         with open(path) as f:
             t = json.load(f)
-        return t["mean_per_launch_bytes"]
-    except (OSError, KeyError, ValueError):
-        return None
+        return t.get("mean_per_launch_bytes"), t.get("mean_valu_insts_per_cell_pair")
+    except (OSError, ValueError):
+        return None, None
 
 
 TUNE_BUCKETS = (96, 192, 384, 768, 1536, 3072, 5000)
@@ -372,12 +374,13 @@ def main():
 
     # dominant kernel: k_scan (one launch per call: long pairs, then lane blocks)
     tops_achieved = OPS_PER_CELL * cells / (kernel_ms / 1e3) / 1e12 if kernel_ms > 0 else 0.0
+    traffic, insts_per_pair = measured_counters()
     roofline = {
         "bound": "valu", "kernel": "k_scan",
         "achieved": round(tops_achieved, 3), "peak": round(PEAK_INT16_TOPS, 1),
         "unit": "TOP/s (packed int16 lane-ops, 9 per cell)",
         "frac": round(tops_achieved / PEAK_INT16_TOPS, 4),
-        "traffic": measured_traffic(),
+        "traffic": traffic,
         "kernel_gcups": round(cells / (kernel_ms / 1e3) / 1e9, 1),
         "avg_launch_ms": round(kernel_ms / (args.steps * len(queries)), 2),
     }
@@ -386,6 +389,14 @@ def main():
         gbs = roofline["traffic"] / (roofline["avg_launch_ms"] / 1e3) / 1e9
         roofline["hbm"] = {"achieved": round(gbs, 1), "peak": 8000.0, "unit": "GB/s",
                            "frac": round(gbs / 8000.0, 4)}
+    if insts_per_pair and kernel_ms > 0:
+        # the issue side: one v_pk_minimum3_f16 does two of the 9 ops, so the
+        # op count above can pass the peak; instructions issued cannot
+        tips = cells / 2 * insts_per_pair / (kernel_ms / 1e3) / 1e12
+        roofline["issue"] = {"valu_insts_per_cell_pair": insts_per_pair,
+                             "achieved": round(tips, 3), "peak": round(PEAK_VALU_TIPS, 2),
+                             "unit": "T VALU lane-instructions/s",
+                             "frac": round(tips / PEAK_VALU_TIPS, 4)}
     pq_out = {str(k): {"gcups": round(v["cells"] / (v["total_ms"] / 1e3) / 1e9, 1),
                        "kernel_gcups": round(v["cells"] / (v["kernel_ms"] / 1e3) / 1e9, 1),
                        "kernel_ms": round(v["kernel_ms"] / args.steps, 2)}

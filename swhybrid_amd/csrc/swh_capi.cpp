@@ -373,17 +373,17 @@ static int tuned_group_lanes(const swh_ctx* c, int qlen, int Wg, const swh_searc
 }
 
 // Built-in cost per padded query column of each lane configuration (order of
-// kLaneCfgs), measured on MI355X over a 5M-sequence database: the pair table
-// at W=24 is 1-1.2% cheaper than at W=16 from query length ~1000 on, but
-// 3-6% dearer below (fewer waves per SIMD hide less of the per-strip set-up);
-// the single-residue profile costs 6-12% more.
-static double builtin_cost(int k, int qlen) {
+// kLaneCfgs), measured on MI355X over a 5M-sequence database at query lengths
+// 144-5478: the pair table at W=24 is 1.8-3.5% cheaper per padded column than
+// at W=16 (fewer H/E round trips through LDS per cell); the single-residue
+// profile costs 3-11% more.
+static double builtin_cost(int k) {
     switch (k) {
-        case 0: return 1.0;                           // (16, pair)
-        case 1: return qlen >= 768 ? 0.990 : 1.03;    // (24, pair)
-        case 2: return 1.115;                         // (16, single)
-        case 3: return 1.066;                         // (24, single)
-        default: return 1.056;                        // (32, single)
+        case 0: return 1.0;      // (16, pair)
+        case 1: return 0.978;    // (24, pair)
+        case 2: return 1.11;     // (16, single)
+        case 3: return 1.05;     // (24, single)
+        default: return 1.03;    // (32, single)
     }
 }
 
@@ -405,7 +405,7 @@ Shape choose_shape(const swh_ctx* c, const int* qlens, int nq, const swh_search_
         double t = 0;
         for (int i = 0; i < nq; ++i) {
             const int b = tune_bucket(c, qlens[i]);
-            double w = builtin_cost(k, qlens[i]);
+            double w = builtin_cost(k);
             if (b >= 0) {
                 const float m = c->tune[b].lane_cost[k];
                 w = m > 0 ? m : 1e30;   // not measured: never preferred
@@ -443,7 +443,8 @@ int enqueue_tasks(swh_ctx* c, swh_db* db, const Task* t, int nt, const Shape& sh
                                                : exact_thr;
     const int gap_oe = c->gap_open + c->gap_extend;
     std::vector<QueryDesc> hd(3 * (size_t)nt);
-    int items_l = 0, items_g = 0, max_qpad_l = 0, max_pass_g = 0, max_gpw = 1, max_qpad_r = 0;
+    int items_l = 0, items_g = 0, max_qpad_l = 0, max_pass_g = 0, max_gpw = 1, max_pass_r = 0;
+    int max_gpw_r = 0;   // lane groups per wave of the multi-pass rescores (0: none)
     bool lane_border = false, group_border = false;
     for (int i = 0; i < nt; ++i) {
         const Task& T = t[i];
@@ -484,9 +485,10 @@ int enqueue_tasks(swh_ctx* c, swh_db* db, const Task* t, int nt, const Shape& sh
         }
         QueryDesc& Rd = hd[2 * nt + i];
         Rd = base;
-        Rd.qpad = ((T.qlen + 7) / 8) * 8;
-        Rd.group = 1;
-        max_qpad_r = std::max(max_qpad_r, Rd.qpad);
+        Rd.qpad = ((T.qlen + kRescoreWidth - 1) / kRescoreWidth) * kRescoreWidth;
+        Rd.group = std::min(kWave, Rd.qpad / kRescoreWidth);
+        max_pass_r = std::max(max_pass_r, Rd.group * kRescoreWidth);
+        if (Rd.qpad > Rd.group * kRescoreWidth) max_gpw_r = std::max(max_gpw_r, kWave / Rd.group);
         st->cells += (uint64_t)T.qlen *
                      (range_residues(db, T.pair_begin, T.pair_end) +
                       range_residues(db, T.lane_begin, T.lane_end));
@@ -569,14 +571,17 @@ int enqueue_tasks(swh_ctx* c, swh_db* db, const Task* t, int nt, const Shape& sh
     if (per_wg)
         grid = (int)std::min<uint64_t>(grid, std::max<uint64_t>(1, c->border_budget / per_wg));
     grid = std::max(1, std::min(grid, items));
-    // rescore: one sequence per lane, grid capped so its borders fit 1 GiB
-    const int ps_r = std::min(max_qpad_r, kSegCols) + kProfPad;
+    // rescore: a lane group per sequence, one pass's profile columns in LDS;
+    // only queries swept in more than one pass keep a border per group, and
+    // the grid is capped so those borders fit 1 GiB
+    const int ps_r = max_pass_r + kProfPad;
     const size_t lds_r = (size_t)kProfRows * ps_r * sizeof(int16_t);
     const uint32_t lmax = std::max(db->max_rows_lane, db->max_rows_pair);
-    const uint64_t rstride = (uint64_t)lmax * kWave;
+    const uint64_t rstride = (uint64_t)max_gpw_r * lmax;
     const uint64_t r_per_wg = rstride * sizeof(uint2) * kWavesPerWG;
     int rgrid = c->num_cus * rescore_max_wg_per_cu(lds_r);
-    rgrid = (int)std::max<uint64_t>(1, std::min<uint64_t>(rgrid, (size_t(1) << 30) / r_per_wg));
+    if (r_per_wg)
+        rgrid = (int)std::max<uint64_t>(1, std::min<uint64_t>(rgrid, (size_t(1) << 30) / r_per_wg));
     HIP_TRY(lr.border->reserve(
         std::max<uint64_t>({16, per_wg * (uint64_t)grid, r_per_wg * (uint64_t)rgrid})));
 
@@ -611,6 +616,7 @@ int enqueue_tasks(swh_ctx* c, swh_db* db, const Task* t, int nt, const Shape& sh
     r.work_counter = counters + 2;
     r.border = (uint2*)lr.border->p;
     r.border_stride = rstride;
+    r.border_rows = lmax;
     r.n_queries = nt;
     r.n_lane_blocks_first = db->n_pairs;
     r.ps = ps_r;

@@ -13,6 +13,7 @@ constexpr int kProfRows = 24;        // residue codes 0..23 (23 = pad)
 constexpr int kSegCols = 1344;       // query columns per LDS profile segment (multiple of 16, 24 and 32;
                                      // 24 rows x (1344+8) int16 = 64,896 B of LDS)
 constexpr int kProfPad = 8;          // extra int16 per LDS row: rows start 16 B apart mod 128 B
+constexpr int kRescoreWidth = 8;    // int32 rescore: query columns per lane and pass
 constexpr uint32_t kNoSeq = 0xffffffffu;
 // The int16 scan is exact for scores below this (its three-way max reads the
 // biased patterns as fp16 values, which order only below 0x7C00); sequences
@@ -80,10 +81,12 @@ struct RescoreArgs {
     const uint32_t* perm;
     const uint32_t* sorted_len;   // real length per sorted position
     const uint32_t* sorted_blk;   // block of each sorted position
-    const QueryDesc* queries;     // qpad = qlen rounded to 8; items from the overflow counts
+    const QueryDesc* queries;     // qpad = qlen rounded to 8, group = lanes per sequence;
+                                  // items from the overflow counts
     uint32_t* work_counter;
-    uint2* border;
+    uint2* border;                // per wave slot, border_stride uint2
     uint64_t border_stride;
+    uint32_t border_rows;         // rows reserved per lane group (longest sequence)
     int n_queries;
     int n_lane_blocks_first;      // index of the first lane-mode block
     int ps;

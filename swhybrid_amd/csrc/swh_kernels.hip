@@ -549,8 +549,14 @@ __global__ __launch_bounds__(256) void k_scan(ScanArgs A) {
 }
 
 // ---------------------------------------------------------------------------
-// int32 rescore: one overflowed sequence per lane, same strip scheme with
-// 32-bit saturating arithmetic biased by INT32_MIN.
+// int32 rescore of the overflowed sequences, in the pipeline of the group
+// items: G lanes share one sequence (G = the query's rescore group, up to 64),
+// lane g holds query columns [pass*G*8 + g*8, +8) and at step t works on row
+// t-g; the (H, F) of its last column reaches lane g+1 through the wave-wide
+// DPP shift.  32-bit saturating arithmetic biased by INT32_MIN.  A sequence
+// of length n costs ceil(qpad / (8G)) passes of n + G - 1 steps on one wave,
+// so a few long overflowed sequences no longer run their whole matrix on one
+// lane each.
 template <int W>
 __device__ __forceinline__ void single_profile(int (&P)[W], const short* lds, int ps, int a,
                                                int jj0) {
@@ -559,8 +565,8 @@ __device__ __forceinline__ void single_profile(int (&P)[W], const short* lds, in
     for (int c = 0; c < W; ++c) P[c] = pa[c];
 }
 
-// Rescore work items are 256 overflow-list entries; query q owns
-// ceil(count_q / 256) of them.  Thread 0 walks the (short) query table to map
+// Rescore work items: kWavesPerWG * (64 / G) overflow-list entries of one
+// query (one per lane group).  Thread 0 walks the (short) query table to map
 // a claimed item to its query, so no host round trip sizes this launch.
 __device__ __forceinline__ bool next_rescore_item(const RescoreArgs& A, int* s_buf, int& local,
                                                   int& q) {
@@ -569,7 +575,8 @@ __device__ __forceinline__ bool next_rescore_item(const RescoreArgs& A, int* s_b
         int it = (int)atomicAdd(A.work_counter, 1u);
         int found = -1;
         for (int j = 0; j < A.n_queries; ++j) {
-            const int n = (int)((*A.queries[j].ovf_count + 255u) >> 8);
+            const uint32_t per_item = kWavesPerWG * (kWave / A.queries[j].group);
+            const int n = (int)((*A.queries[j].ovf_count + per_item - 1) / per_item);
             if (it < n) {
                 found = j;
                 break;
@@ -586,19 +593,25 @@ __device__ __forceinline__ bool next_rescore_item(const RescoreArgs& A, int* s_b
 }
 
 __global__ __launch_bounds__(256) void k_rescore(RescoreArgs A) {
-    constexpr int W = 8;   // divides every padded query length (all are multiples of 8)
+    constexpr int W = kRescoreWidth;
     extern __shared__ __align__(16) short lds[];
     __shared__ int s_buf[2];
+    __shared__ int s_red[kWavesPerWG][kWave];
     const int wave = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
-    uint2* bord = A.border + (size_t)(blockIdx.x * kWavesPerWG + wave) * A.border_stride;
-    int loaded_q = -1, loaded_seg = -1;
+    uint2* slot = A.border + (size_t)(blockIdx.x * kWavesPerWG + wave) * A.border_stride;
     int local, q;
     while (next_rescore_item(A, s_buf, local, q)) {
         const QueryDesc Qd = A.queries[q];
         const int n = (int)*Qd.ovf_count;
-        const int entry = local * 256 + threadIdx.x;
-        const bool valid = entry < n;
+        const int G = Qd.group;
+        const int gpw = kWave / G;
+        const int gi = lane / G;
+        const int gl = lane - gi * G;
+        const bool in_group = gi < gpw;
+        const int entry = (local * kWavesPerWG + wave) * gpw + gi;
+        const bool valid = in_group && entry < n;
+        uint2* bord = slot + (size_t)(in_group ? gi : 0) * A.border_rows;
         uint32_t s = 0, len = 0;
         uint64_t base = 0;
         uint32_t stride = 0;
@@ -616,61 +629,91 @@ __global__ __launch_bounds__(256) void k_rescore(RescoreArgs A) {
                 stride = 2;
             }
         }
-        // rows of the wave = longest member, rounded to a multiple of 4
-        uint32_t lmax = len;
-        for (int off = 32; off > 0; off >>= 1) lmax = max(lmax, (uint32_t)__shfl_xor((int)lmax, off));
-        lmax = __builtin_amdgcn_readfirstlane(lmax);
-        const int quads = (int)((lmax + 3) >> 2);
-        const int nseg = (Qd.qpad + kSegCols - 1) / kSegCols;
+        // rows of the wave = its longest sequence (equal in every lane: scalar loops)
+        uint32_t rmax = len;
+        for (int off = 32; off > 0; off >>= 1) rmax = max(rmax, (uint32_t)__shfl_xor((int)rmax, off));
+        rmax = __builtin_amdgcn_readfirstlane(rmax);
+        const int rows = (int)len;
+        const int pass_cols = G * W;
+        const int npass = (Qd.qpad + pass_cols - 1) / pass_cols;
         int S = kZero32;
-        for (int seg = 0; seg < nseg; ++seg) {
-            const int seg0 = seg * kSegCols;
-            const int segw = min(kSegCols, Qd.qpad - seg0);
-            if (q != loaded_q || seg != loaded_seg) {
-                __syncthreads();
-                stage_profile(lds, Qd.prof, Qd.pstride, A.ps, seg0, segw);
-                __syncthreads();
-                loaded_q = q;
-                loaded_seg = seg;
+        for (int pass = 0; pass < npass; ++pass) {
+            const int col_base = pass * pass_cols;
+            const int pcols = min(pass_cols, Qd.qpad - col_base);
+            const int nl = pcols / W;
+            __syncthreads();
+            stage_profile(lds, Qd.prof, Qd.pstride, A.ps, col_base, pcols);
+            __syncthreads();
+            const bool has_in = pass > 0;
+            const bool has_out = pass + 1 < npass;
+            const bool on = valid && gl < nl;
+            const int jj0 = gl * W;
+            int h[W], e[W], hq[W];
+#pragma unroll
+            for (int c = 0; c < W; ++c) {
+                h[c] = kZero32;
+                e[c] = kZero32;
+                hq[c] = kZero32;
             }
-            for (int c0 = seg0; c0 < seg0 + segw; c0 += W) {
-                const bool has_in = c0 > 0;
-                const bool has_out = c0 + W < Qd.qpad;
-                int h[W], e[W], hq[W];
+            int diag0 = kZero32;
+            int ch = kZero32, cf = kZero32;   // this lane's last (H, F)
+            const int steps = rmax > 0 ? (int)rmax + nl - 1 : 0;
+            // residue words and border entries are loaded 4 steps ahead
+            const bool reads_border = has_in && gl == 0 && valid;
+            uint32_t wn[4];
+            uint2 bn[4];
 #pragma unroll
-                for (int c = 0; c < W; ++c) {
-                    h[c] = kZero32;
-                    e[c] = kZero32;
-                    hq[c] = kZero32;
+            for (int u = 0; u < 4; ++u) {
+                const int row = u - gl;
+                wn[u] = 0u;
+                bn[u] = make_uint2((uint32_t)kZero32, (uint32_t)kZero32);
+                if (on && row >= 0 && row < rows) wn[u] = A.words[base + (uint64_t)(row >> 2) * stride];
+                if (reads_border && u < rows) bn[u] = bord[u];
+            }
+            for (int t0 = 0; t0 < steps; t0 += 4) {
+                uint32_t wc[4];
+                uint2 bc[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    wc[u] = wn[u];
+                    bc[u] = bn[u];
+                    const int tn = t0 + 4 + u;
+                    const int row = tn - gl;
+                    if (on && row >= 0 && row < rows)
+                        wn[u] = A.words[base + (uint64_t)(row >> 2) * stride];
+                    if (reads_border && tn < rows) bn[u] = bord[tn];
                 }
-                int diag0 = kZero32;
-                for (int k = 0; k < quads; ++k) {
-                    uint32_t w = 0x17171717u;   // pad code 23 beyond the sequence
-                    if ((uint32_t)(4 * k) < len) w = A.words[base + (uint64_t)k * stride];
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        const int row = 4 * k + r;
-                        int a = (w >> (8 * r)) & 0xff;
-                        if ((uint32_t)row >= len) a = 23;
+                for (int u = 0; u < 4; ++u) {
+                    const int row = t0 + u - gl;
+                    // every lane takes part in the shift (no lane may be masked off here)
+                    const int hx = (int)wave_shr1((uint32_t)ch);
+                    const int fx = (int)wave_shr1((uint32_t)cf);
+                    const int hl_u = gl == 0 ? (int)bc[u].x : hx;
+                    const int fl_u = gl == 0 ? (int)bc[u].y : fx;
+                    if (on && row >= 0 && row < rows) {
+                        const int a = (wc[u] >> (8 * (row & 3))) & 0xff;
                         int P[W];
-                        single_profile<W>(P, lds, A.ps, a, c0 - seg0);
-                        uint2 bin = make_uint2((uint32_t)kZero32, (uint32_t)kZero32);
-                        if (has_in) bin = bord[(size_t)row * kWave + lane];
+                        single_profile<W>(P, lds, A.ps, a, jj0);
                         int fout;
-                        row_cells<W, int>(h, e, hq, P, diag0, (int)bin.x, (int)bin.y,
-                                          A.gap_oe, A.gap_e, fout);
+                        row_cells<W, int>(h, e, hq, P, diag0, hl_u, fl_u, A.gap_oe, A.gap_e,
+                                          fout);
                         S = row_max_into<W, int>(S, h);
-                        if (has_out)
-                            bord[(size_t)row * kWave + lane] =
-                                make_uint2((uint32_t)h[W - 1], (uint32_t)fout);
-                        diag0 = (int)bin.x;
+                        ch = h[W - 1];
+                        cf = fout;
+                        diag0 = hl_u;
+                        if (has_out && gl == nl - 1) bord[row] = make_uint2((uint32_t)ch, (uint32_t)cf);
                     }
                 }
             }
         }
-        if (valid) {
+        s_red[wave][lane] = S;
+        __syncthreads();
+        if (valid && gl == 0) {
+            int m = S;
+            for (int k = 1; k < G; ++k) m = vmax(m, s_red[wave][lane + k]);
             const uint32_t idx = A.perm[s];
-            if (idx != kNoSeq) Qd.scores[idx] = (int)((int64_t)S - (int64_t)kZero32);
+            if (idx != kNoSeq) Qd.scores[idx] = (int)((int64_t)m - (int64_t)kZero32);
         }
     }
 }
