@@ -70,7 +70,8 @@ typedef struct swh_search_opts {
     int group_lanes;          /* lanes cooperating on one long sequence pair
                                  (1..64); 0 = auto */
     int group_width;          /* query columns per lane in group mode: 0 = auto,
-                                 else 8 or 16 */
+                                 else 8, 16 or 24 (24 with 24-column lane
+                                 strips only) */
     int use_graph;            /* capture the launch sequence in a hipGraph and
                                  replay it while the query lengths, database and
                                  options stay the same */
@@ -81,7 +82,8 @@ typedef struct swh_search_opts {
 /* Database layout options.  Zero-initialise for defaults. */
 typedef struct swh_db_opts {
     uint32_t group_min_len;   /* sequences at least this long are stored as
-                                 pairs for group mode; 0 = default (2048),
+                                 pairs for group mode; 0 = default (total
+                                 residues / (2560 x CUs), at least 2048),
                                  0xffffffff = never */
 } swh_db_opts;
 
@@ -154,7 +156,7 @@ typedef struct swh_tune_entry {
     int strip_width;          /* best lane-scan tile depth (16, 24, 32) */
     int profile_mode;         /* best profile placement (SWH_PROFILE_SINGLE / _PAIR) */
     int group_lanes;          /* best lanes per long sequence pair (1..64) */
-    int group_width;          /* best group-mode tile depth (8, 16) */
+    int group_width;          /* best group-mode tile depth (8, 16, 24) */
     float lane_cost[SWH_TUNE_LANE_CONFIGS];   /* measured launch ms per padded
                                                  query column, per configuration */
 } swh_tune_entry;

@@ -124,6 +124,20 @@ int swh_ctx_set_scoring(swh_ctx* c, const int32_t* matrix, int gap_open, int gap
     return SWH_OK;
 }
 
+// Default group-mode threshold: a lane block runs its sequences' rows through
+// the whole query on one wave, so sequences longer than about 0.4 of a
+// sequence slot's share of the database (total residues / (2 sequences x 256
+// lanes x 2 workgroups x CUs)) go to group mode, where G lanes share a pair;
+// at least 2048.  On a 5M-sequence, 1.75G-residue database this is 2,672, on
+// a 2M-sequence, 4.3G-residue one with a tail to 35,000 residues 6,540 (the
+// factor 0.4 measured best of 0.33, 0.4 and 0.5 over those two).
+static uint32_t default_group_min_len(const swh_ctx* c, const uint32_t* lengths, uint64_t n) {
+    uint64_t total = 0;
+    for (uint64_t i = 0; i < n; ++i) total += lengths[i];
+    const uint64_t t = total / ((uint64_t)std::max(c->num_cus, 1) * 2560u);
+    return (uint32_t)std::min<uint64_t>(std::max<uint64_t>(t, 2048u), 1u << 20);
+}
+
 int swh_db_create(swh_ctx* c, const uint8_t* residues, const uint64_t* offsets,
                   const uint32_t* lengths, uint64_t nseq, const swh_db_opts* opts,
                   swh_db** out) {
@@ -132,13 +146,15 @@ int swh_db_create(swh_ctx* c, const uint8_t* residues, const uint64_t* offsets,
     if (nseq > 0 && (!residues || !offsets || !lengths))
         return fail(SWH_ERR_ARG, "residues/offsets/lengths is NULL");
     if (nseq >= (uint64_t)kNoSeq / 2) return fail(SWH_ERR_ARG, "too many sequences");
-    uint32_t group_min = (opts && opts->group_min_len) ? opts->group_min_len : 2048u;
+    uint32_t group_min = (opts && opts->group_min_len) ? opts->group_min_len
+                                                       : default_group_min_len(c, lengths, nseq);
     HIP_TRY(hipSetDevice(c->device));
     static std::atomic<uint64_t> next_uid{1};
     auto* db = new swh_db();
     db->ctx = c;
     db->uid = next_uid.fetch_add(1);
     db->nseq = nseq;
+    db->group_min_len = group_min;
     // sort by length, longest first (ties keep caller order): the work queues
     // hand out the longest blocks first
     std::vector<uint32_t> order(nseq);
@@ -302,21 +318,20 @@ uint64_t range_padded(const swh_db* db, int b0, int b1) {
     return b1 > b0 ? db->blk_padded[b1] - db->blk_padded[b0] : 0;
 }
 
-// Group mode shape for one query: lanes per group G for a given column width
-// W.  A group covers the whole query in one pass when it can (G = strips, at
-// most 64); the score is the useful fraction of lane-columns times the share
-// of a lane's step spent on cells (about 13 VALU ops of per-step overhead
-// beside the 9.5 per packed cell).
-static double group_eff(int qlen, int W, int req_g, int* G_out) {
-    const int qp = ((qlen + W - 1) / W) * W;
-    const int strips = qp / W;
-    const int G = req_g > 0 ? std::min(req_g, kWave) : std::min(strips, kWave);
+// Group mode shape for one query: lanes per group G at column width W.  A
+// group covers the query in as few passes as it can (G = strips, at most 64):
+// a pair's item lasts passes x its rows, and the longest pairs' items must
+// not outlast the launch.
+int group_lanes_for(int qlen, int W) { return std::min((qlen + W - 1) / W, kWave); }
+
+// Useful share of a wave's lane-columns in group mode at width W with
+// G = group_lanes_for: strips / (passes x G) of a pass, (64 / G) x G / 64 of
+// the wave.
+static double group_lane_use(int qlen, int W) {
+    const int strips = (qlen + W - 1) / W;
+    const int G = group_lanes_for(qlen, W);
     const int passes = (strips + G - 1) / G;
-    const int gpw = kWave / G;
-    const double lane_use =
-        (double)strips / ((double)passes * G) * (double)(gpw * G) / kWave;
-    *G_out = G;
-    return lane_use * (9.0 * W) / (9.5 * W + 13.0) * ((double)qlen / qp);
+    return (double)strips / ((double)passes * G) * (double)((kWave / G) * G) / kWave;
 }
 
 // Profile row stride of a query: a multiple of 96 covers every strip width
@@ -332,8 +347,10 @@ void build_profile(const int32_t* matrix, const uint8_t* q, int qlen, int16_t* o
 }
 
 int check_opts(const swh_search_opts& o) {
-    if (o.group_width && o.group_width != 8 && o.group_width != 16)
-        return fail(SWH_ERR_ARG, "group_width must be 8 or 16");
+    if (o.group_width && o.group_width != 8 && o.group_width != 16 && o.group_width != 24)
+        return fail(SWH_ERR_ARG, "group_width must be 8, 16 or 24");
+    if (o.group_width == 24 && o.strip_width && o.strip_width != 24)
+        return fail(SWH_ERR_ARG, "group_width 24 runs with strip_width 24 only");
     if (o.group_lanes < 0 || o.group_lanes > kWave)
         return fail(SWH_ERR_ARG, "group_lanes must be in 1..64");
     if (o.strip_width && o.strip_width != 16 && o.strip_width != 24 && o.strip_width != 32)
@@ -391,15 +408,16 @@ static double builtin_cost(int k) {
 // profile placement with the least predicted time, summed over the queries as
 // cost per padded query column x padded columns.  Costs come from the tuning
 // table when one is installed, else from builtin_cost.  Group scan: the
-// width of the bucket holding most of the batch's columns, else 16 (on the
-// 5M-sequence database 8 columns per lane lose 2% at query length 144, where
-// the long pairs' single pass is shortest).
+// width of the bucket holding most of the batch's columns, else 16 or 24 by
+// lane use (24 is instantiated with 24-column lanes only; 8 columns per lane
+// lose 2% on the 5M-sequence database at query length 144).
 Shape choose_shape(const swh_ctx* c, const int* qlens, int nq, const swh_search_opts& o) {
     Shape sh{16, o.group_width, true};
     double best = 1e300;
     for (int k = 0; k < SWH_TUNE_LANE_CONFIGS; ++k) {
         const LaneCfg& L = kLaneCfgs[k];
         if (o.strip_width && o.strip_width != L.width) continue;
+        if (o.group_width == 24 && L.width != 24) continue;   // built with 24-column lanes only
         if (o.profile_mode == SWH_PROFILE_PAIR && !L.pair) continue;
         if (o.profile_mode == SWH_PROFILE_SINGLE && L.pair) continue;
         double t = 0;
@@ -419,13 +437,24 @@ Shape choose_shape(const swh_ctx* c, const int* qlens, int nq, const swh_search_
         }
     }
     if (!sh.Wg) {
+        // 16, or 24 next to 24-column lanes where its lane use, weighted by its
+        // 10% faster step (measured at equal lane use), covers more columns
         sh.Wg = 16;
+        if (sh.Wl == 24) {
+            double u16 = 0, u24 = 0;
+            for (int i = 0; i < nq; ++i) {
+                u16 += qlens[i] * group_lane_use(qlens[i], 16);
+                u24 += qlens[i] * group_lane_use(qlens[i], 24) * 1.10;
+            }
+            if (u24 > u16) sh.Wg = 24;
+        }
         if (!c->tune.empty()) {
             std::vector<double> cols(c->tune.size(), 0.0);
             for (int i = 0; i < nq; ++i) cols[tune_bucket(c, qlens[i])] += qlens[i];
             const size_t b = std::max_element(cols.begin(), cols.end()) - cols.begin();
             sh.Wg = c->tune[b].group_width;
         }
+        if (sh.Wg == 24 && sh.Wl != 24) sh.Wg = 16;   // a bucket tuned with other lanes
     }
     return sh;
 }
@@ -469,7 +498,8 @@ int enqueue_tasks(swh_ctx* c, swh_db* db, const Task* t, int nt, const Shape& sh
         QueryDesc& Gd = hd[nt + i];
         Gd = base;
         Gd.qpad = ((T.qlen + sh.Wg - 1) / sh.Wg) * sh.Wg;
-        group_eff(T.qlen, sh.Wg, tuned_group_lanes(c, T.qlen, sh.Wg, o), &Gd.group);
+        const int req_g = tuned_group_lanes(c, T.qlen, sh.Wg, o);
+        Gd.group = req_g > 0 ? std::min(req_g, kWave) : group_lanes_for(T.qlen, sh.Wg);
         Gd.blk_begin = T.pair_begin;
         Gd.blk_end = T.pair_end;
         const int per_item = kWavesPerWG * (kWave / Gd.group);

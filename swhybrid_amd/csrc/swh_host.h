@@ -168,6 +168,7 @@ struct swh_db {
     int n_pairs = 0;            // group-mode pair blocks, blocks [0, n_pairs)
     int n_blocks = 0;           // all blocks; lane blocks are [n_pairs, n_blocks)
     uint32_t max_rows_lane = 0, max_rows_pair = 0;
+    uint32_t group_min_len = 0;   // sequences at least this long are in pair blocks
     uint64_t padded_pair = 0, padded_lane = 0;   // padded residues per layout
     uint64_t residues_pair = 0;                  // real residues in pair blocks
     std::vector<BlockDesc> blocks;
@@ -208,6 +209,8 @@ uint64_t range_padded(const swh_db* db, int b0, int b1);
 int check_opts(const swh_search_opts& o);
 int check_query(const uint8_t* q, int qlen, int i);
 Shape choose_shape(const swh_ctx* c, const int* qlens, int nq, const swh_search_opts& o);
+// Lanes per group in group mode: the fewest passes over the query.
+int group_lanes_for(int qlen, int W);
 
 // What one launch sequence owns: its stream, descriptor staging and strip
 // borders.  Two of them let two launch sequences be in flight at once.

@@ -75,23 +75,25 @@ __device__ __forceinline__ void stage_profile(short* lds, const int16_t* __restr
 }
 
 // One row of a strip: W cells, packed pairs.  P[c] = (s(q_c, a), s(q_c, b)).
-// h/e/hq hold H(i-1,c), E(i-1,c), H(i-1,c)-(open+ext) on entry and row i on exit.
+// h holds H(i-1,c) on entry and H(i,c) on exit; e holds E(i,c) on entry and
+// E(i+1,c) on exit, so H(i,c) - (open+ext) is used where it is made and never
+// kept across rows.
 template <int W, typename T>
-__device__ __forceinline__ void row_cells(T (&h)[W], T (&e)[W], T (&hq)[W], const T (&P)[W],
-                                          T diag0, T hl, T fl, T Q, T R, T& fout) {
+__device__ __forceinline__ void row_cells(T (&h)[W], T (&e)[W], const T (&P)[W], T diag0, T hl,
+                                          T fl, T Q, T R, T& fout) {
     T X = adds(diag0, P[0]);      // H(i-1, c0-1) + s
     T f = fl;                      // F(i, c0-1)
     T hqp = subs(hl, Q);           // H(i, c0-1) - (open+ext)
 #pragma unroll
     for (int c = 0; c < W; ++c) {
         f = vmax(subs(f, R), hqp);                 // F(i, c)
-        e[c] = vmax(subs(e[c], R), hq[c]);         // E(i, c)
         T Xn = X;
         if (c + 1 < W) Xn = adds(h[c], P[c + 1]);  // diagonal term of column c+1
-        T hn = vmax3(X, e[c], f);                  // H(i, c); zero floor by saturation
-        hq[c] = subs(hn, Q);
+        const T hn = vmax3(X, e[c], f);            // H(i, c); zero floor by saturation
+        const T hq = subs(hn, Q);
+        e[c] = vmax(subs(e[c], R), hq);            // E(i+1, c)
         h[c] = hn;
-        hqp = hq[c];
+        hqp = hq;
         X = Xn;
     }
     fout = f;
@@ -251,12 +253,11 @@ __device__ __forceinline__ void lane_strip(const uint2* __restrict__ wds, int qu
                                            uint2* __restrict__ bord, const Src& src, s2 Q, s2 R,
                                            s2& S, int lane) {
     const s2 Z = as_s2(kZero16x2);
-    s2 h[W], e[W], hq[W];
+    s2 h[W], e[W];
 #pragma unroll
     for (int c = 0; c < W; ++c) {
         h[c] = Z;
         e[c] = Z;
-        hq[c] = Z;
     }
     s2 diag0 = Z;
     const uint2 zz = make_uint2(kZero16x2, kZero16x2);
@@ -293,7 +294,7 @@ __device__ __forceinline__ void lane_strip(const uint2* __restrict__ wds, int qu
             const s2 hl = as_s2(bc[r].x);
             const s2 fl = as_s2(bc[r].y);
             s2 fout;
-            row_cells<W, s2>(h, e, hq, P, diag0, hl, fl, Q, R, fout);
+            row_cells<W, s2>(h, e, P, diag0, hl, fl, Q, R, fout);
             S = row_max_into<W, s2>(S, h);
             if (OUT) bo[r * kWave] = make_uint2(bits(h[W - 1]), bits(fout));
             diag0 = hl;
@@ -435,27 +436,34 @@ __device__ __forceinline__ void group_item(const ScanArgs& A, const QueryDesc& Q
         const bool has_in = pass > 0;
         const bool has_out = pass + 1 < npass;
         const bool on = has_pair && gl < nl;
-        const int jj0 = gl * W;
-        s2 h[W], e[W], hq[W];
+        const int jj0 = on ? gl * W : 0;   // idle lanes load staged columns too
+        s2 h[W], e[W];
 #pragma unroll
         for (int c = 0; c < W; ++c) {
             h[c] = Z;
             e[c] = Z;
-            hq[c] = Z;
         }
         s2 diag0 = Z;
         uint32_t ch = kZero16x2, cf = kZero16x2;   // this lane's last (H, F)
         const int steps = rmax > 0 ? (int)rmax + nl - 1 : 0;
-        // residue words and border entries are loaded 4 steps ahead
+        // residue words and border entries are loaded 4 steps ahead; rows
+        // outside the pair read a clamped word (any code is a valid LDS row)
         const bool reads_border = has_in && gl == 0 && has_pair;
+        const int rows = (int)bd.rows;
+        const int last_row = max(rows - 1, 0);
         uint2 wn[4], bn[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            const int row = u - gl;
-            wn[u] = make_uint2(0u, 0u);
+            wn[u] = pw[min(max(u - gl, 0), last_row) >> 2];
             bn[u] = make_uint2(kZero16x2, kZero16x2);
-            if (on && row >= 0 && row < (int)bd.rows) wn[u] = pw[row >> 2];
-            if (reads_border && u < (int)bd.rows) bn[u] = bord[u];
+            if (reads_border && u < rows) bn[u] = bord[u];
+        }
+        // the profile rows of a step are loaded while the step before it computes
+        RawProfile<W> rp;
+        {
+            const int sh = 8 * ((-gl) & 3);
+            load_profile<W>(rp, lds, A.ps_group, (wn[0].x >> sh) & 0xff, (wn[0].y >> sh) & 0xff,
+                            jj0);
         }
         for (int t0 = 0; t0 < steps; t0 += 4) {
             uint2 wc[4], bc[4];
@@ -463,33 +471,35 @@ __device__ __forceinline__ void group_item(const ScanArgs& A, const QueryDesc& Q
             for (int u = 0; u < 4; ++u) {
                 wc[u] = wn[u];
                 bc[u] = bn[u];
+                wn[u] = pw[min(max(t0 + 4 + u - gl, 0), last_row) >> 2];
                 const int tn = t0 + 4 + u;
-                const int row = tn - gl;
-                if (on && row >= 0 && row < (int)bd.rows) wn[u] = pw[row >> 2];
-                if (reads_border && tn < (int)bd.rows) bn[u] = bord[tn];
+                if (reads_border && tn < rows) bn[u] = bord[tn];
             }
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 const int row = t0 + u - gl;
+                const RawProfile<W> cur = rp;
+                const uint2 nw = u < 3 ? wc[u + 1] : wn[0];
+                const int sh = 8 * ((row + 1) & 3);
+                load_profile<W>(rp, lds, A.ps_group, (nw.x >> sh) & 0xff, (nw.y >> sh) & 0xff,
+                                jj0);
                 // every lane takes part in the shift (no lane may be masked off here)
                 const uint32_t hx = wave_shr1(ch);
                 const uint32_t fx = wave_shr1(cf);
                 const uint32_t hl_u = gl == 0 ? bc[u].x : hx;
                 const uint32_t fl_u = gl == 0 ? bc[u].y : fx;
-                if (on && row >= 0 && row < (int)bd.rows) {
-                    const int sh = 8 * (row & 3);
-                    const int a = (wc[u].x >> sh) & 0xff;
-                    const int b = (wc[u].y >> sh) & 0xff;
+                if (on && row >= 0 && row < rows) {
                     s2 P[W];
-                    pair_profile<W>(P, lds, A.ps_group, a, b, jj0);
+                    interleave_profile<W>(P, cur);
                     s2 fout;
-                    row_cells<W, s2>(h, e, hq, P, diag0, as_s2(hl_u), as_s2(fl_u), Q, R, fout);
+                    row_cells<W, s2>(h, e, P, diag0, as_s2(hl_u), as_s2(fl_u), Q, R, fout);
                     S = row_max_into<W, s2>(S, h);
                     ch = bits(h[W - 1]);
                     cf = bits(fout);
                     diag0 = as_s2(hl_u);
                     if (has_out && gl == nl - 1) bord[row] = make_uint2(ch, cf);
                 }
+                __builtin_amdgcn_sched_barrier(0);
             }
         }
     }
@@ -525,8 +535,11 @@ __device__ __forceinline__ bool next_scan_item(const ScanArgs& A, int* s_buf, in
     return local >= 0;
 }
 
+// WL = 16: at most 168 VGPRs, so that three workgroups (12 waves) share a CU
+// as its LDS allows; wider strips run two.
 template <int WL, int WG, bool PAIR>
-__global__ __launch_bounds__(256) void k_scan(ScanArgs A) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WL == 16 ? 3 : 2)))
+void k_scan(ScanArgs A) {
     static_assert(!PAIR || WL == 16 || WL == 24, "pair table only for W = 16, 24");
     extern __shared__ __align__(16) short lds[];
     __shared__ int s_buf[3];
@@ -648,12 +661,11 @@ __global__ __launch_bounds__(256) void k_rescore(RescoreArgs A) {
             const bool has_out = pass + 1 < npass;
             const bool on = valid && gl < nl;
             const int jj0 = gl * W;
-            int h[W], e[W], hq[W];
+            int h[W], e[W];
 #pragma unroll
             for (int c = 0; c < W; ++c) {
                 h[c] = kZero32;
                 e[c] = kZero32;
-                hq[c] = kZero32;
             }
             int diag0 = kZero32;
             int ch = kZero32, cf = kZero32;   // this lane's last (H, F)
@@ -696,8 +708,7 @@ __global__ __launch_bounds__(256) void k_rescore(RescoreArgs A) {
                         int P[W];
                         single_profile<W>(P, lds, A.ps, a, jj0);
                         int fout;
-                        row_cells<W, int>(h, e, hq, P, diag0, hl_u, fl_u, A.gap_oe, A.gap_e,
-                                          fout);
+                        row_cells<W, int>(h, e, P, diag0, hl_u, fl_u, A.gap_oe, A.gap_e, fout);
                         S = row_max_into<W, int>(S, h);
                         ch = h[W - 1];
                         cf = fout;
@@ -784,7 +795,8 @@ static hipError_t launch_scan_w(const ScanArgs& a, int grid, size_t lds, hipStre
 
 #define SWH_SCAN_CASES(X)                                                                  \
     X(16, 8, true) X(16, 16, true) X(24, 8, true) X(24, 16, true) X(16, 8, false)          \
-    X(16, 16, false) X(24, 8, false) X(24, 16, false) X(32, 8, false) X(32, 16, false)
+    X(16, 16, false) X(24, 8, false) X(24, 16, false) X(32, 8, false) X(32, 16, false)     \
+    X(24, 24, true) X(24, 24, false)
 
 hipError_t launch_scan(int wl, int wg, bool pair, const ScanArgs& a, int grid, size_t lds,
                        hipStream_t s) {

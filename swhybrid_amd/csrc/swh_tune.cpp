@@ -71,7 +71,8 @@ int swh_set_tuning(swh_ctx* c, const swh_tune_entry* e, int n) {
             return fail(SWH_ERR_ARG, "entry %d: pair table needs strip width 16 or 24", i);
         if (t.group_lanes < 0 || t.group_lanes > kWave)
             return fail(SWH_ERR_ARG, "entry %d: group_lanes %d", i, t.group_lanes);
-        if (t.group_width != 8 && t.group_width != 16)
+        if (t.group_width != 8 && t.group_width != 16 &&
+            !(t.group_width == 24 && t.strip_width == 24))
             return fail(SWH_ERR_ARG, "entry %d: group_width %d", i, t.group_width);
         bool any = false;
         for (float v : t.lane_cost) {
@@ -233,11 +234,15 @@ int swh_autotune(swh_ctx* c, const uint8_t* residues, const uint64_t* offsets,
         E.group_width = 16;
         if (!has_pairs) continue;
         best = 1e300;
-        for (int wg : {8, 16}) {
+        std::vector<int> widths = {8, 16};
+        if (E.strip_width == 24) widths.push_back(24);
+        for (int wg : widths) {
             const int strips = (L + wg - 1) / wg;
-            std::vector<int> cand = {std::min(strips, kWave)};
+            std::vector<int> cand = {group_lanes_for(L, wg)};
             for (int g : {8, 16, 32})
-                if (g < strips && g != cand[0]) cand.push_back(g);
+                if (g < strips) cand.push_back(g);
+            std::sort(cand.begin(), cand.end());
+            cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
             for (int g : cand) {
                 swh_search_opts o{};
                 o.strip_width = E.strip_width;

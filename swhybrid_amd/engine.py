@@ -87,7 +87,8 @@ class Engine:
 
     def load_db(self, db: SeqDB, group_min_len: int = 0) -> Database:
         """Pack and upload; sequences of at least group_min_len residues are
-        stored as pairs for group mode (0 = default 2048, -1 = never)."""
+        stored as pairs for group mode (0 = default: total residues / (2560 x
+        CUs), at least 2048; -1 = never)."""
         res = np.ascontiguousarray(db.residues, dtype=np.uint8)
         off = np.ascontiguousarray(db.offsets, dtype=np.uint64)
         lens = np.ascontiguousarray(db.lengths, dtype=np.uint32)

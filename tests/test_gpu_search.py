@@ -178,7 +178,7 @@ def test_group_mode_auto(eng, qlen):
 
 
 @pytest.mark.parametrize("lanes,width", [(1, 8), (3, 16), (7, 8), (16, 16), (63, 8), (64, 16),
-                                         (64, 8)])
+                                         (64, 8), (5, 24), (21, 24), (64, 24)])
 def test_group_mode_shapes(eng, lanes, width):
     db = long_db(400 + lanes, n=40)
     q = G.queries([1234], 401)[0]
@@ -282,7 +282,8 @@ def test_autotune_installs_a_table_and_scores_stay_exact():
     assert [t["qlen_max"] for t in ents] == [64, 400, 1500]
     for t in ents:
         assert t["strip_width"] in (16, 24, 32) and t["profile_mode"] in (1, 2)
-        assert t["group_width"] in (8, 16) and 0 <= t["group_lanes"] <= 64
+        assert t["group_width"] in (8, 16, 24) and 0 <= t["group_lanes"] <= 64
+        assert t["group_width"] != 24 or t["strip_width"] == 24
         assert all(c > 0 for c in t["lane_cost"])
     assert e.get_tuning() == ents
     d = e.load_db(db, group_min_len=res["group_min_len"] if res["group_min_len"] < 2**31 else -1)
@@ -327,14 +328,28 @@ def test_tuning_table_validation():
     good = dict(qlen_max=100, strip_width=16, profile_mode=2, group_lanes=0, group_width=16,
                 lane_cost=[1, 1, 1, 1, 1])
     for bad in (dict(strip_width=20), dict(profile_mode=0), dict(group_width=12),
+                dict(group_width=24),                      # 24 needs 24-column lanes
                 dict(strip_width=32, profile_mode=2), dict(lane_cost=[0, 0, 0, 0, 0]),
                 dict(group_lanes=65)):
         with pytest.raises(Exception):
             e.set_tuning([{**good, **bad}])
     with pytest.raises(Exception):
         e.set_tuning([good, {**good, "qlen_max": 50}])     # not ascending
+    e.set_tuning([good, {**good, "qlen_max": 200, "strip_width": 24, "group_width": 24}])
     e.set_tuning([good])
     e.close()
+
+
+def test_group_width_24_needs_24_column_lanes(eng):
+    db = long_db(77, n=20)
+    d = eng.load_db(db, group_min_len=300)
+    q = G.queries([500], 78)[0]
+    with pytest.raises(Exception):
+        eng.search(d, q, group_width=24, strip_width=16)
+    a, st = eng.search(d, q, group_width=24)
+    b, _ = eng.search(d, q, group_width=16)
+    assert st["cells_group"] > 0 and np.array_equal(a, b)
+    d.close()
 
 
 # --- extremes ------------------------------------------------------------------------
