@@ -281,14 +281,17 @@ __device__ __forceinline__ void lane_strip(const uint2* __restrict__ wds, int qu
             for (int r = 0; r < 4; ++r) bn[r] = bp[r * kWave];
         }
         uint2* bo = bord + (size_t)k * 4 * kWave + lane;
-        // the profile of row r+1 is loaded while row r computes; row barriers
-        // keep the scheduler from hoisting every load of the tile at once
+        // the profile of row r+1 is loaded while row r computes: the barrier
+        // after the loads keeps the scheduler from sinking them to the end of
+        // the row (where the next row would wait for them), the one at the end
+        // of the row from hoisting every load of the tile at once
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const typename Src::Raw cur = rp;
             const uint2 nw = r < 3 ? w : wn;
             const int sh = 8 * ((r + 1) & 3);
             src.load(rp, (nw.x >> sh) & 0xff, (nw.y >> sh) & 0xff);
+            __builtin_amdgcn_sched_barrier(0);
             s2 P[W];
             src.pairs(P, cur);
             const s2 hl = as_s2(bc[r].x);
@@ -483,6 +486,7 @@ __device__ __forceinline__ void group_item(const ScanArgs& A, const QueryDesc& Q
                 const int sh = 8 * ((row + 1) & 3);
                 load_profile<W>(rp, lds, A.ps_group, (nw.x >> sh) & 0xff, (nw.y >> sh) & 0xff,
                                 jj0);
+                __builtin_amdgcn_sched_barrier(0);   // issue the loads before the step's work
                 // every lane takes part in the shift (no lane may be masked off here)
                 const uint32_t hx = wave_shr1(ch);
                 const uint32_t fx = wave_shr1(cf);
