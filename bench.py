@@ -257,9 +257,11 @@ def run_dbfile(eng):
 
 def run_hybrid(db, qlens=(1000, 5478)):
     """The reference's hybrid scheduling: the worker pipeline over the same
-    5M-sequence database held in host memory, chunks of 256 MB to the
+    5M-sequence database held in host memory, chunks of up to 512 MB to the
     HIPWorker and of 4 MB to the CPU worker (the host's AVX2 cores), all
-    from one DataPool; HIPWorker alone first, then both."""
+    from one DataPool; HIPWorker alone first, then both.  `*_gcups` is the
+    reference's own measure (first worker start to last stop, after the
+    first chunk is filled); `*_wall_gcups` times the whole call."""
     from swhybrid_amd import datagen as G
     from swhybrid_amd.engine import hybrid_search
     # the HIPWorker's own host work (chunk copies, upload) competes with the
@@ -274,12 +276,16 @@ def run_hybrid(db, qlens=(1000, 5478)):
         b = hybrid_search(db, q, k=TOP_K, hip_workers=1, cpu_threads=threads)
         same = bool((a["top_score"] == b["top_score"]).all())
         ra, rb = a["report"], b["report"]
+        cells = db.total * L
         out[str(L)] = {"hip_only_gcups": round(ra["gcups"], 1), "hybrid_gcups": round(rb["gcups"], 1),
+                       "hip_only_wall_gcups": round(cells / ra["wall_seconds"] / 1e9, 1),
+                       "hybrid_wall_gcups": round(cells / rb["wall_seconds"] / 1e9, 1),
                        "cpu_share": round(rb["cpu_residues"] / max(1, rb["cpu_residues"] + rb["hip_residues"]), 4),
                        "cpu_worker_gcups": round(rb["cpu_residues"] * L / rb["cpu_busy_s"] / 1e9, 1)
                        if rb["cpu_busy_s"] > 0 else 0.0,
                        "top20_scores_equal": same}
-        log(f"hybrid qlen {L}: HIP alone {ra['gcups']:.0f}, HIP + CPU {rb['gcups']:.0f} GCUPS "
+        log(f"hybrid qlen {L}: HIP alone {ra['gcups']:.0f} ({out[str(L)]['hip_only_wall_gcups']:.0f} "
+            f"wall), HIP + CPU {rb['gcups']:.0f} ({out[str(L)]['hybrid_wall_gcups']:.0f} wall) GCUPS "
             f"(CPU share {out[str(L)]['cpu_share']:.3f})")
     return out
 

@@ -21,8 +21,8 @@ extern "C" {
 const char* swhy_last_error(void);
 
 /* Search one query through the worker pipeline.
- *   chunk_bytes    DataPool chunk budget per fill (0 = 256 MB, the reference's
- *                  CUDAWorker bufferSize, src/CUDAWorker.cu:44)
+ *   chunk_bytes    DataPool chunk budget per fill (0 = 512 MB, twice the
+ *                  reference's CUDAWorker bufferSize, src/CUDAWorker.cu:44)
  *   n_workers      HIPWorkers on devices 0..n_workers-1
  *   out_top_*      topk entries: score, caller index, pool index; ties are
  *                  ordered by pool index (the order sequences are handed out),
@@ -46,7 +46,7 @@ typedef struct swhy_pool_opts {
     int topk;
     int hip_workers;           /* HIPWorkers on devices 0..hip_workers-1 */
     int cpu_threads;           /* threads of the CPU worker; 0 = no CPU worker */
-    uint64_t chunk_bytes;      /* HIPWorker chunk (0 = 256 MB) */
+    uint64_t chunk_bytes;      /* HIPWorker chunk (0 = 512 MB) */
     uint64_t cpu_chunk_bytes;  /* CPU worker chunk (0 = 4 MB); the CPU worker takes no
                                   chunk once only hip_workers x chunk_bytes are left */
 } swhy_pool_opts;
@@ -57,6 +57,9 @@ typedef struct swhy_pool_report {
     uint64_t hip_residues, cpu_residues;   /* padded residues each kind searched */
     double hip_busy_s, cpu_busy_s;         /* time each kind spent computing */
     uint64_t recalc;           /* sequences rescored after the workers */
+    double wall_seconds;       /* the whole call: worker set-up, the first chunk's
+                                  fill and pack (outside `seconds`, as in the
+                                  reference's timing), search, merge */
 } swhy_pool_report;
 
 /* swhy_pool_search with a CPU worker beside the HIPWorkers.  out_top_* get

@@ -183,7 +183,8 @@ class SwhyPoolOpts(C.Structure):
 class SwhyPoolReport(C.Structure):
     _fields_ = [("seconds", C.c_double), ("gcups", C.c_double), ("hip_residues", C.c_uint64),
                 ("cpu_residues", C.c_uint64), ("hip_busy_s", C.c_double),
-                ("cpu_busy_s", C.c_double), ("recalc", C.c_uint64)]
+                ("cpu_busy_s", C.c_double), ("recalc", C.c_uint64),
+                ("wall_seconds", C.c_double)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

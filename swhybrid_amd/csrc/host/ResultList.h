@@ -1,12 +1,14 @@
 // ResultList.h -- fixed-size top-k list.
 //
 // Semantics of the reference's ResultList<int> (src/ResultList.h:18-92): the
-// list always holds k entries, starting as k (0, 0) results; a push whose
-// score is <= the bottom score is rejected; an accepted push enters before the
-// first entry that ranks below it and the last entry drops out.  Entries with
-// equal scores are ordered by index, ascending: for pushes in increasing index
-// order that is the reference's first-pushed order, and it keeps merges of
-// several workers' lists independent of thread timing.
+// list always holds k entries, starting as k (0, 0) results; an accepted push
+// enters before the first entry that ranks below it and the last entry drops
+// out.  Entries with equal scores are ordered by index, ascending, and a push
+// is rejected unless it ranks above the last entry.  For pushes in increasing
+// index order that is the reference's rule (reject a score <= the bottom
+// score, first-pushed order among ties); it also keeps merges of several
+// workers' lists, whose indices interleave, independent of thread timing (a
+// tie with the bottom entry and a smaller index enters).
 #pragma once
 #include <cstddef>
 #include <cstdio>
@@ -30,7 +32,7 @@ class ResultList {
 
     void push(const T* scores, size_t n, size_t baseIdx) {
         for (size_t i = 0; i < n; ++i)
-            if (scores[i] > bottom_) push(scores[i], baseIdx + i);
+            if (scores[i] >= bottom_) push(scores[i], baseIdx + i);
     }
 
     void mergeList(const ResultList<T>& other) {
@@ -38,7 +40,7 @@ class ResultList {
     }
 
     void push(T score, size_t idx) {
-        if (size_ == 0 || score <= bottom_) return;
+        if (size_ == 0 || score <= 0 || !ranks_below(results_[size_ - 1], score, idx)) return;
         size_t pos = 0;
         while (pos < size_ && !ranks_below(results_[pos], score, idx)) ++pos;
         if (pos == size_) return;

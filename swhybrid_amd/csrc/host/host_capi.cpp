@@ -4,6 +4,7 @@
 // a database in memory or in the reference's .seq/.map/.title files; the
 // DBmaker; and FASTA query input.
 #include <algorithm>
+#include <chrono>
 #include <cstring>
 #include <exception>
 #include <memory>
@@ -18,6 +19,15 @@
 
 namespace {
 thread_local std::string g_err;
+// HIPWorker chunk: twice the reference's 256 MB (src/CUDAWorker.cu:44).  One
+// launch per chunk, and on a 256-CU device a launch's tail is a smaller share
+// of a larger chunk; the first chunk's fill is not overlapped with a search,
+// so beyond a point larger chunks lose (5M-sequence pipeline, whole-call
+// GCUPS at query length 1000 / 5478: 4,726 / 6,688 at 256 MB, 4,982 / 7,417
+// at 512 MB, 4,612 / 7,650 at 1 GB).  Starting with smaller chunks loses more:
+// the pool hands out the longest sequences first, and a small launch of them
+// is mostly tail (4,658 / 6,161 from 64 MB doubling to 512 MB).
+const size_t kDefaultChunk = size_t(512) << 20;
 }
 
 namespace {
@@ -33,8 +43,9 @@ int run_pool(DataPool& pool, uint64_t nseq, const int32_t* matrix, int gap_open,
     if (out_scores) sink.assign(nseq, -1);
     std::vector<Result<int>> top;
     swhy_pool_report r{};
+    const auto t0 = std::chrono::steady_clock::now();
     {
-        WorkerPool wp(&pool, &params, o.chunk_bytes ? o.chunk_bytes : (size_t(256) << 20),
+        WorkerPool wp(&pool, &params, o.chunk_bytes ? o.chunk_bytes : kDefaultChunk,
                       o.cpu_chunk_bytes);
         const int rc = wp.run(out_scores ? sink.data() : nullptr);
         if (rc != 0) {
@@ -50,6 +61,7 @@ int run_pool(DataPool& pool, uint64_t nseq, const int32_t* matrix, int gap_open,
         r.recalc = wp.recalcCount();
         top = wp.result().getResult();
     }
+    r.wall_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     for (int t = 0; t < o.topk; ++t) {
         const bool real = t < (int)top.size() && top[t].score > 0;
         out_top_score[t] = t < (int)top.size() ? top[t].score : 0;
