@@ -132,22 +132,31 @@ int swh_ctx_set_scoring(swh_ctx* c, const int32_t* matrix, int gap_open, int gap
 // a 2M-sequence, 4.3G-residue one with a tail to 35,000 residues 6,540 (the
 // factor 0.4 measured best of 0.33, 0.4 and 0.5 over those two).
 static uint32_t default_group_min_len(const swh_ctx* c, const uint32_t* lengths, uint64_t n) {
+    const uint64_t cus = (uint64_t)std::max(c->num_cus, 1);
     uint64_t total = 0;
     for (uint64_t i = 0; i < n; ++i) total += lengths[i];
-    const uint64_t t = total / ((uint64_t)std::max(c->num_cus, 1) * 2560u);
+    const uint64_t t = total / (cus * 2560u);
     return (uint32_t)std::min<uint64_t>(std::max<uint64_t>(t, 2048u), 1u << 20);
 }
 
-int swh_db_create(swh_ctx* c, const uint8_t* residues, const uint64_t* offsets,
-                  const uint32_t* lengths, uint64_t nseq, const swh_db_opts* opts,
-                  swh_db** out) {
-    if (!c || !out) return fail(SWH_ERR_ARG, "ctx/out is NULL");
+// A database of fewer than 4,096 sequences per CU cannot give every workgroup
+// slot (2 per CU, 512 sequences per lane item) 4 lane items of one query, so a
+// single query leaves most CUs idle.  Such a database also gets a second
+// layout in which every sequence of at least 128 residues is a pair, split
+// over G lanes, for launches with few lane items (100k sequences, 29.9M
+// residues, query length 256: 848 GCUPS on the lane layout, 5,882 on this
+// one; 64 queries in one launch: 8,920 on the lane layout, 5,572 on this one).
+constexpr uint64_t kSmallDbSeqsPerCu = 4096;
+constexpr uint32_t kSmallDbGroupMinLen = 128;
+
+}  // extern "C"
+
+// One layout of the database: sequences of at least group_min residues in
+// pair blocks, the rest in lane blocks.
+static int create_layout(swh_ctx* c, const uint8_t* residues, const uint64_t* offsets,
+                         const uint32_t* lengths, uint64_t nseq, uint32_t group_min,
+                         swh_db** out) {
     *out = nullptr;
-    if (nseq > 0 && (!residues || !offsets || !lengths))
-        return fail(SWH_ERR_ARG, "residues/offsets/lengths is NULL");
-    if (nseq >= (uint64_t)kNoSeq / 2) return fail(SWH_ERR_ARG, "too many sequences");
-    uint32_t group_min = (opts && opts->group_min_len) ? opts->group_min_len
-                                                       : default_group_min_len(c, lengths, nseq);
     HIP_TRY(hipSetDevice(c->device));
     static std::atomic<uint64_t> next_uid{1};
     auto* db = new swh_db();
@@ -284,8 +293,37 @@ int swh_db_create(swh_ctx* c, const uint8_t* residues, const uint64_t* offsets,
     return SWH_OK;
 }
 
+extern "C" {
+
+int swh_db_create(swh_ctx* c, const uint8_t* residues, const uint64_t* offsets,
+                  const uint32_t* lengths, uint64_t nseq, const swh_db_opts* opts,
+                  swh_db** out) {
+    if (!c || !out) return fail(SWH_ERR_ARG, "ctx/out is NULL");
+    *out = nullptr;
+    if (nseq > 0 && (!residues || !offsets || !lengths))
+        return fail(SWH_ERR_ARG, "residues/offsets/lengths is NULL");
+    if (nseq >= (uint64_t)kNoSeq / 2) return fail(SWH_ERR_ARG, "too many sequences");
+    const bool auto_layout = !opts || !opts->group_min_len;
+    const uint32_t group_min = auto_layout ? default_group_min_len(c, lengths, nseq)
+                                           : opts->group_min_len;
+    swh_db* db = nullptr;
+    int rc = create_layout(c, residues, offsets, lengths, nseq, group_min, &db);
+    if (rc) return rc;
+    if (auto_layout && nseq > 0 && nseq < (uint64_t)std::max(c->num_cus, 1) * kSmallDbSeqsPerCu &&
+        group_min > kSmallDbGroupMinLen) {
+        rc = create_layout(c, residues, offsets, lengths, nseq, kSmallDbGroupMinLen, &db->alt);
+        if (rc) {
+            swh_db_destroy(db);
+            return rc;
+        }
+    }
+    *out = db;
+    return SWH_OK;
+}
+
 int swh_db_destroy(swh_db* db) {
     if (!db) return SWH_OK;
+    if (db->alt) swh_db_destroy(db->alt);
     if (db->ctx) (void)hipSetDevice(db->ctx->device);
     db->words.release();
     db->d_blocks.release();
@@ -300,9 +338,12 @@ int swh_db_info(const swh_db* db, uint64_t* nseq, uint64_t* residues, uint64_t* 
     if (!db) return fail(SWH_ERR_ARG, "db is NULL");
     if (nseq) *nseq = db->nseq;
     if (residues) *residues = db->residues;
-    if (device_bytes)
-        *device_bytes = db->words.bytes + db->d_blocks.bytes + db->perm.bytes +
-                        db->sorted_len.bytes + db->sorted_blk.bytes;
+    if (device_bytes) {
+        *device_bytes = 0;
+        for (const swh_db* d = db; d; d = d->alt)
+            *device_bytes += d->words.bytes + d->d_blocks.bytes + d->perm.bytes +
+                             d->sorted_len.bytes + d->sorted_blk.bytes;
+    }
     return SWH_OK;
 }
 
@@ -659,9 +700,19 @@ int enqueue_tasks(swh_ctx* c, swh_db* db, const Task* t, int nt, const Shape& sh
 
 // Search every query over the whole database in one launch sequence.
 // scores_host gets nq rows of nseq.
-int run_queries(swh_ctx* c, swh_db* db, const uint8_t* const* qs, const int* qlens,
+// The layout a launch of nq queries runs on: the small-database pair layout
+// when the lane layout would give fewer than 4 lane items per workgroup slot.
+static swh_db* pick_layout(const swh_ctx* c, swh_db* db, int nq) {
+    if (!db->alt) return db;
+    const uint64_t items =
+        (uint64_t)nq * ((uint64_t)(db->n_blocks - db->n_pairs) + kWavesPerWG - 1) / kWavesPerWG;
+    return items < 4u * 2u * (uint64_t)std::max(c->num_cus, 1) ? db->alt : db;
+}
+
+int run_queries(swh_ctx* c, swh_db* db_in, const uint8_t* const* qs, const int* qlens,
                        int nq, const swh_search_opts& o, int32_t* scores_host,
                        swh_stats* stats) {
+    swh_db* db = pick_layout(c, db_in, nq);
     const double t_start = now_ms();
     swh_stats st{};
     int rc = check_opts(o);

@@ -169,6 +169,7 @@ struct swh_db {
     int n_blocks = 0;           // all blocks; lane blocks are [n_pairs, n_blocks)
     uint32_t max_rows_lane = 0, max_rows_pair = 0;
     uint32_t group_min_len = 0;   // sequences at least this long are in pair blocks
+    swh_db* alt = nullptr;        // small databases: the same sequences with more pairs
     uint64_t padded_pair = 0, padded_lane = 0;   // padded residues per layout
     uint64_t residues_pair = 0;                  // real residues in pair blocks
     std::vector<BlockDesc> blocks;

@@ -340,6 +340,33 @@ def test_tuning_table_validation():
     e.close()
 
 
+def test_small_database_second_layout(eng):
+    """A database too small to fill the device with lane items of one query
+    carries a second layout with every sequence of 128+ residues in lane
+    groups: few-query launches run on it, launches with many lane items on the
+    lane layout, and both give the oracle's scores."""
+    rng = np.random.Generator(np.random.PCG64(91))
+    db = G.SeqDB.from_lengths(rng, G.lognormal_lengths(rng, 4000, 300, 0.7, 20, 1900))
+    d = eng.load_db(db)                                   # default layout choice
+    q = G.queries([300], 92)[0]
+    one, st = eng.search(d, q)
+    assert st["cells_group"] > 0                          # no sequence reaches 2048
+    assert np.array_equal(one, O.search_i32(q, db, eng.matrix, eng.gap_open, eng.gap_extend))
+    qs = [G.random_residues(rng, 24) for _ in range(260)]  # 260 x 8 lane items
+    many, stb = eng.search_batch(d, qs)
+    assert stb["cells_group"] == 0
+    for i in (0, 77, 259):
+        assert np.array_equal(many[i], eng.search(d, qs[i])[0])
+        assert np.array_equal(many[i], O.search_i32(qs[i], db, eng.matrix, eng.gap_open,
+                                                     eng.gap_extend))
+    single = eng.load_db(db, group_min_len=2048)          # explicit: one layout
+    assert single.info()["device_bytes"] < d.info()["device_bytes"]
+    _, st1 = eng.search(single, q)
+    assert st1["cells_group"] == 0
+    single.close()
+    d.close()
+
+
 def test_group_width_24_needs_24_column_lanes(eng):
     db = long_db(77, n=20)
     d = eng.load_db(db, group_min_len=300)
