@@ -83,8 +83,10 @@ typedef struct swh_search_opts {
 typedef struct swh_db_opts {
     uint32_t group_min_len;   /* sequences at least this long are stored as
                                  pairs for group mode; 0 = default (total
-                                 residues / (2560 x CUs), at least 2048),
-                                 0xffffffff = never */
+                                 residues / (2560 x CUs), at least 2048; below
+                                 4096 sequences per CU a second layout with
+                                 pairs from 128 residues serves launches of
+                                 few queries), 0xffffffff = never */
 } swh_db_opts;
 
 const char* swh_last_error(void);
