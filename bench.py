@@ -167,9 +167,11 @@ def run_small_db(eng):
     q1 = G.c1_query()
     for _ in range(3):
         eng.search(d, q1)
-    t0 = time.perf_counter()
-    _, st1 = eng.search(d, q1)
-    one = time.perf_counter() - t0
+    one = 1e30
+    for _ in range(5):                                  # a 2 ms call: best of 5
+        t0 = time.perf_counter()
+        _, st1 = eng.search(d, q1)
+        one = min(one, time.perf_counter() - t0)
     d.close()
     log(f"small database: plain {cells / plain / 1e9:.0f} GCUPS, batched "
         f"{st['cells'] / batched / 1e9:.0f} GCUPS, its own query {st1['cells'] / one / 1e9:.0f} GCUPS")

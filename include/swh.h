@@ -84,9 +84,13 @@ typedef struct swh_db_opts {
     uint32_t group_min_len;   /* sequences at least this long are stored as
                                  pairs for group mode; 0 = default (total
                                  residues / (2560 x CUs), at least 2048; below
-                                 4096 sequences per CU a second layout with
+                                 1024 sequences per CU a second layout with
                                  pairs from 128 residues serves launches of
                                  few queries), 0xffffffff = never */
+    uint32_t queries_per_launch;   /* with the default layout: queries the caller
+                                      will search per call, 0 = not known.  When
+                                      known, a small database gets only the one
+                                      layout those calls run on */
 } swh_db_opts;
 
 const char* swh_last_error(void);

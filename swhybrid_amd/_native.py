@@ -93,7 +93,7 @@ class SwhServerStats(C.Structure):
 
 
 class SwhDbOpts(C.Structure):
-    _fields_ = [("group_min_len", C.c_uint32)]
+    _fields_ = [("group_min_len", C.c_uint32), ("queries_per_launch", C.c_uint32)]
 
 
 class SwhTuneEntry(C.Structure):

@@ -71,8 +71,10 @@ void HIPWorker::packDB() {
         }
     if (dbs_[slot]) swh_db_destroy(dbs_[slot]);
     dbs_[slot] = nullptr;
+    swh_db_opts dopt{};
+    dopt.queries_per_launch = 1;   // one query per search: one layout per chunk
     if (swh_db_create(ctx_, reinterpret_cast<const uint8_t*>(dbSeqBuf), offsets_.data(),
-                      lengths_.data(), lengths_.size(), nullptr, &dbs_[slot]) != SWH_OK)
+                      lengths_.data(), lengths_.size(), &dopt, &dbs_[slot]) != SWH_OK)
         throw swh_fail("swh_db_create");
     scores_[slot].resize(lengths_.size());
     e->setEntry(dbSeqBuf, nullptr, scores_[slot].data());

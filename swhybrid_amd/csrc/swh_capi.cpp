@@ -139,15 +139,23 @@ static uint32_t default_group_min_len(const swh_ctx* c, const uint32_t* lengths,
     return (uint32_t)std::min<uint64_t>(std::max<uint64_t>(t, 2048u), 1u << 20);
 }
 
-// A database of fewer than 4,096 sequences per CU cannot give every workgroup
-// slot (2 per CU, 512 sequences per lane item) 4 lane items of one query, so a
-// single query leaves most CUs idle.  Such a database also gets a second
-// layout in which every sequence of at least 128 residues is a pair, split
-// over G lanes, for launches with few lane items (100k sequences, 29.9M
+// A database of fewer than 1,024 sequences per CU cannot give every workgroup
+// slot (2 per CU, 512 sequences per lane item) one lane item of a query, so a
+// single query leaves CUs idle.  Such a database also gets a second layout in
+// which every sequence of at least 128 residues is a pair, split over G lanes,
+// for launches with fewer lane items than slots (100k sequences, 29.9M
 // residues, query length 256: 848 GCUPS on the lane layout, 5,882 on this
-// one; 64 queries in one launch: 8,920 on the lane layout, 5,572 on this one).
-constexpr uint64_t kSmallDbSeqsPerCu = 4096;
+// one; 64 queries in one launch: 8,920 on the lane layout, 5,572 on this one;
+// a 512 MB worker chunk of the 5M-sequence database's longest 500k sequences,
+// 977 lane items, runs 6% faster on the lane layout).
+constexpr uint64_t kSmallDbSeqsPerCu = 1024;
 constexpr uint32_t kSmallDbGroupMinLen = 128;
+
+// At least one lane item (4 lane blocks) per workgroup slot, 2 slots per CU.
+static bool enough_lane_items(const swh_ctx* c, uint64_t lane_blocks, int nq) {
+    const uint64_t items = (uint64_t)nq * ((lane_blocks + kWavesPerWG - 1) / kWavesPerWG);
+    return items >= 2u * (uint64_t)std::max(c->num_cus, 1);
+}
 
 }  // extern "C"
 
@@ -304,13 +312,24 @@ int swh_db_create(swh_ctx* c, const uint8_t* residues, const uint64_t* offsets,
         return fail(SWH_ERR_ARG, "residues/offsets/lengths is NULL");
     if (nseq >= (uint64_t)kNoSeq / 2) return fail(SWH_ERR_ARG, "too many sequences");
     const bool auto_layout = !opts || !opts->group_min_len;
-    const uint32_t group_min = auto_layout ? default_group_min_len(c, lengths, nseq)
-                                           : opts->group_min_len;
+    uint32_t group_min = auto_layout ? default_group_min_len(c, lengths, nseq)
+                                     : opts->group_min_len;
+    bool second = auto_layout && nseq > 0 &&
+                  nseq < (uint64_t)std::max(c->num_cus, 1) * kSmallDbSeqsPerCu &&
+                  group_min > kSmallDbGroupMinLen;
+    if (second && opts && opts->queries_per_launch > 0) {
+        // the caller's launches all pick the same layout: build only that one
+        uint64_t n_lane = 0;
+        for (uint64_t i = 0; i < nseq; ++i) n_lane += lengths[i] < group_min;
+        const uint64_t blocks = (n_lane + kLaneSeqs - 1) / kLaneSeqs;
+        if (!enough_lane_items(c, blocks, (int)opts->queries_per_launch))
+            group_min = kSmallDbGroupMinLen;
+        second = false;
+    }
     swh_db* db = nullptr;
     int rc = create_layout(c, residues, offsets, lengths, nseq, group_min, &db);
     if (rc) return rc;
-    if (auto_layout && nseq > 0 && nseq < (uint64_t)std::max(c->num_cus, 1) * kSmallDbSeqsPerCu &&
-        group_min > kSmallDbGroupMinLen) {
+    if (second) {
         rc = create_layout(c, residues, offsets, lengths, nseq, kSmallDbGroupMinLen, &db->alt);
         if (rc) {
             swh_db_destroy(db);
@@ -701,12 +720,10 @@ int enqueue_tasks(swh_ctx* c, swh_db* db, const Task* t, int nt, const Shape& sh
 // Search every query over the whole database in one launch sequence.
 // scores_host gets nq rows of nseq.
 // The layout a launch of nq queries runs on: the small-database pair layout
-// when the lane layout would give fewer than 4 lane items per workgroup slot.
+// when the lane layout would give fewer lane items than workgroup slots.
 static swh_db* pick_layout(const swh_ctx* c, swh_db* db, int nq) {
     if (!db->alt) return db;
-    const uint64_t items =
-        (uint64_t)nq * ((uint64_t)(db->n_blocks - db->n_pairs) + kWavesPerWG - 1) / kWavesPerWG;
-    return items < 4u * 2u * (uint64_t)std::max(c->num_cus, 1) ? db->alt : db;
+    return enough_lane_items(c, (uint64_t)(db->n_blocks - db->n_pairs), nq) ? db : db->alt;
 }
 
 int run_queries(swh_ctx* c, swh_db* db_in, const uint8_t* const* qs, const int* qlens,

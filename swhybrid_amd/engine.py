@@ -85,15 +85,19 @@ class Engine:
                                             int(gap_extend)))
         self.matrix, self.gap_open, self.gap_extend = m, int(gap_open), int(gap_extend)
 
-    def load_db(self, db: SeqDB, group_min_len: int = 0) -> Database:
+    def load_db(self, db: SeqDB, group_min_len: int = 0, queries_per_launch: int = 0) -> Database:
         """Pack and upload; sequences of at least group_min_len residues are
         stored as pairs for group mode (0 = default: total residues / (2560 x
-        CUs), at least 2048; -1 = never)."""
+        CUs), at least 2048; -1 = never).  With the default, a database of
+        fewer than 1024 sequences per CU also gets a layout with pairs from
+        128 residues for launches of few queries, unless queries_per_launch
+        says which one the caller's launches need."""
         res = np.ascontiguousarray(db.residues, dtype=np.uint8)
         off = np.ascontiguousarray(db.offsets, dtype=np.uint64)
         lens = np.ascontiguousarray(db.lengths, dtype=np.uint32)
         h = C.c_void_p()
-        o = N.SwhDbOpts(0xFFFFFFFF if group_min_len < 0 else int(group_min_len))
+        o = N.SwhDbOpts(0xFFFFFFFF if group_min_len < 0 else int(group_min_len),
+                        int(queries_per_launch))
         N.check(N.lib().swh_db_create(self.handle, N.ptr(res), N.ptr(off), N.ptr(lens),
                                       db.n, C.byref(o), C.byref(h)))
         return Database(self, h, db.n, db.total)

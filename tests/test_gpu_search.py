@@ -364,6 +364,11 @@ def test_small_database_second_layout(eng):
     _, st1 = eng.search(single, q)
     assert st1["cells_group"] == 0
     single.close()
+    one_q = eng.load_db(db, queries_per_launch=1)         # only the layout one query uses
+    assert one_q.info()["device_bytes"] < d.info()["device_bytes"]
+    got, st2 = eng.search(one_q, q)
+    assert st2["cells_group"] == st["cells_group"] and np.array_equal(got, one)
+    one_q.close()
     d.close()
 
 
