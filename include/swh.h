@@ -163,8 +163,11 @@ typedef struct swh_tune_entry {
     int profile_mode;         /* best profile placement (SWH_PROFILE_SINGLE / _PAIR) */
     int group_lanes;          /* best lanes per long sequence pair (1..64) */
     int group_width;          /* best group-mode tile depth (8, 16, 24) */
-    float lane_cost[SWH_TUNE_LANE_CONFIGS];   /* measured launch ms per padded
-                                                 query column, per configuration */
+    float lane_cost[SWH_TUNE_LANE_CONFIGS];   /* measured launch ms per lane column
+                                                 (query length rounded up to 8, a
+                                                 narrower last strip weighted by
+                                                 its per-row work), per
+                                                 configuration */
 } swh_tune_entry;
 
 typedef struct swh_tune_result {

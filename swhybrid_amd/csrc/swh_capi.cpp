@@ -449,10 +449,23 @@ static int tuned_group_lanes(const swh_ctx* c, int qlen, int Wg, const swh_searc
     return 0;
 }
 
-// Built-in cost per padded query column of each lane configuration (order of
+// Lane columns a query costs at strip width W: it is computed to its length
+// rounded up to 8, full strips of W and a narrower last strip, which pays a
+// strip row's fixed work (table row loads, border, row maximum) over fewer
+// columns: 8 columns cost 1.30 of a full strip's per column, 16 1.06, 24 1.02
+// (from the instructions per row of each width's loop).
+double lane_cols(int qlen, int W) {
+    static const double kRemCost[4] = {0.0, 1.30, 1.06, 1.02};
+    const int q8 = ((qlen + 7) / 8) * 8;
+    const int full = q8 / W * W;
+    const int rem = q8 - full;
+    return full + rem * kRemCost[rem / 8];
+}
+
+// Built-in cost per lane column of each lane configuration (order of
 // kLaneCfgs), measured on MI355X over a 5M-sequence database at query lengths
-// 144-5478: the pair table at W=24 is 1.8-3.5% cheaper per padded column than
-// at W=16 (fewer H/E round trips through LDS per cell); the single-residue
+// 144-5478: the pair table at W=24 is 1.8-3.5% cheaper per column than at
+// W=16 (fewer H/E round trips through LDS per cell); the single-residue
 // profile costs 3-11% more.
 static double builtin_cost(int k) {
     switch (k) {
@@ -466,7 +479,7 @@ static double builtin_cost(int k) {
 
 // Launch shape shared by the queries of a launch.  Lane scan: the width and
 // profile placement with the least predicted time, summed over the queries as
-// cost per padded query column x padded columns.  Costs come from the tuning
+// cost per lane column x lane columns (lane_cols).  Costs come from the tuning
 // table when one is installed, else from builtin_cost.  Group scan: the
 // width of the bucket holding most of the batch's columns, else 16 or 24 by
 // lane use (24 is instantiated with 24-column lanes only; 8 columns per lane
@@ -488,7 +501,7 @@ Shape choose_shape(const swh_ctx* c, const int* qlens, int nq, const swh_search_
                 const float m = c->tune[b].lane_cost[k];
                 w = m > 0 ? m : 1e30;   // not measured: never preferred
             }
-            t += (double)((qlens[i] + L.width - 1) / L.width) * L.width * w;
+            t += lane_cols(qlens[i], L.width) * w;
         }
         if (t < best) {
             best = t;
@@ -547,7 +560,7 @@ int enqueue_tasks(swh_ctx* c, swh_db* db, const Task* t, int nt, const Shape& sh
         base.ovf_threshold = thr;
         QueryDesc& L = hd[i];
         L = base;
-        L.qpad = ((T.qlen + sh.Wl - 1) / sh.Wl) * sh.Wl;
+        L.qpad = ((T.qlen + 7) / 8) * 8;   // the last strip may be narrower than Wl
         L.group = 1;
         L.blk_begin = T.lane_begin;
         L.blk_end = T.lane_end;
@@ -604,7 +617,8 @@ int enqueue_tasks(swh_ctx* c, swh_db* db, const Task* t, int nt, const Shape& sh
         }
         const QueryDesc& L = hd[list - nt];
         const int b = L.blk_begin + k * kWavesPerWG;
-        return (double)db->blocks[b].rows * (L.qpad / sh.Wl) * (9.5 * sh.Wl + 13.0);
+        const int strips = (L.qpad + sh.Wl - 1) / sh.Wl;
+        return (double)db->blocks[b].rows * (9.5 * L.qpad + 13.0 * strips);
     };
     struct Head {
         double cost;

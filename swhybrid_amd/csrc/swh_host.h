@@ -212,6 +212,8 @@ int check_query(const uint8_t* q, int qlen, int i);
 Shape choose_shape(const swh_ctx* c, const int* qlens, int nq, const swh_search_opts& o);
 // Lanes per group in group mode: the fewest passes over the query.
 int group_lanes_for(int qlen, int W);
+// Cost-weighted lane columns of a query at strip width W.
+double lane_cols(int qlen, int W);
 
 // What one launch sequence owns: its stream, descriptor staging and strip
 // borders.  Two of them let two launch sequences be in flight at once.

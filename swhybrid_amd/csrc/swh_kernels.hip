@@ -28,6 +28,8 @@
 // the device.
 #include "swh_internal.h"
 
+#include <type_traits>
+
 namespace swh {
 
 typedef short s2 __attribute__((ext_vector_type(2)));
@@ -332,11 +334,37 @@ __device__ __forceinline__ void emit(const QueryDesc& Qd, const uint32_t* perm, 
     }
 }
 
+// Calls f with std::integral_constant<int, w> for a strip of wl columns: the
+// kernel's width W, or for the query's last strip a narrower one (wl is a
+// multiple of 8 below W), so a query is computed to its length rounded to 8
+// rather than to W.
+template <int W, typename F>
+__device__ __forceinline__ void with_strip_width(int wl, F&& f) {
+    if (wl == W) {
+        f(std::integral_constant<int, W>{});
+    } else {
+        if constexpr (W > 24) {
+            if (wl == 24) {
+                f(std::integral_constant<int, 24>{});
+                return;
+            }
+        }
+        if constexpr (W > 16) {
+            if (wl == 16) {
+                f(std::integral_constant<int, 16>{});
+                return;
+            }
+        }
+        f(std::integral_constant<int, 8>{});
+    }
+}
+
 // One lane-mode work item: 4 consecutive lane blocks of query q, one per wave.
 // PAIR (W = 16 or 24): per strip the workgroup builds the pair table in LDS;
 // otherwise the single profile is staged per 1344-column segment and each row
 // interleaves two residue rows (a 32-wide pair table would not leave room for
-// two workgroups per CU, so W = 32 always stages the single profile).
+// two workgroups per CU, so W = 32 always stages the single profile).  The
+// last strip is qpad - c0 wide (qpad = qlen rounded up to 8).
 template <int W, bool PAIR>
 __device__ __forceinline__ void lane_item(const ScanArgs& A, const QueryDesc& Qd, int q,
                                           int local, short* lds, uint2* bord, int& loaded_q,
@@ -366,23 +394,31 @@ __device__ __forceinline__ void lane_item(const ScanArgs& A, const QueryDesc& Qd
             }
             if (active) {
                 for (int c0 = seg0; c0 < seg0 + segw; c0 += W) {
-                    const SingleSource<W> src{lds, A.ps_lane, c0 - seg0};
-                    lane_strip_any<W>(wds, quads, bord, src, c0 > 0, c0 + W < Qd.qpad, Q, R, S,
-                                      lane);
+                    const int wl = min(W, seg0 + segw - c0);
+                    with_strip_width<W>(wl, [&](auto wc) {
+                        constexpr int w = decltype(wc)::value;
+                        const SingleSource<w> src{lds, A.ps_lane, c0 - seg0};
+                        lane_strip_any<w>(wds, quads, bord, src, c0 > 0, c0 + w < Qd.qpad, Q,
+                                          R, S, lane);
+                    });
                 }
             }
         }
     } else {
         uint32_t* tab = reinterpret_cast<uint32_t*>(lds);
         for (int c0 = 0; c0 < Qd.qpad; c0 += W) {
-            __syncthreads();
-            build_pair_table<W>(tab, Qd.prof, Qd.pstride, c0);
-            __syncthreads();
-            if (active) {
-                const PairSource<W> src{tab};
-                lane_strip_any<W>(wds, quads, bord, src, c0 > 0, c0 + W < Qd.qpad, Q, R, S,
-                                  lane);
-            }
+            const int wl = min(W, Qd.qpad - c0);
+            with_strip_width<W>(wl, [&](auto wc) {
+                constexpr int w = decltype(wc)::value;
+                __syncthreads();
+                build_pair_table<w>(tab, Qd.prof, Qd.pstride, c0);
+                __syncthreads();
+                if (active) {
+                    const PairSource<w> src{tab};
+                    lane_strip_any<w>(wds, quads, bord, src, c0 > 0, c0 + w < Qd.qpad, Q, R, S,
+                                      lane);
+                }
+            });
         }
         loaded_q = -1;   // the single-profile staging area was overwritten
     }

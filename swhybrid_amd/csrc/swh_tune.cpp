@@ -223,7 +223,7 @@ int swh_autotune(swh_ctx* c, const uint8_t* residues, const uint64_t* offsets,
             double ms;
             if ((rc = timed(h.db, q0, qpb, o, &ms))) return restore(rc);
             const int w = kLaneCfgs[k].width;
-            E.lane_cost[k] = (float)(ms / ((double)qpb * ((L + w - 1) / w) * w));
+            E.lane_cost[k] = (float)(ms / ((double)qpb * lane_cols(L, w)));
             if (ms < best) {
                 best = ms;
                 E.strip_width = w;
