@@ -4,7 +4,9 @@ query lengths are searched in full on the GPU; the oracle checks a sample
 depend on size check everything: every strip width and the batched launch
 give the same scores, the top-k equals a sort of the scores, and a forced
 int32 rescore changes nothing.  The near-copy database (20% planted copies,
-BLOSUM62 10/2 and BLOSUM50 13/2) runs at a tenth of its size."""
+BLOSUM62 10/2 and BLOSUM50 13/2) runs at a tenth of its size, and once at full
+size with BLOSUM50, where every sequence the int16 pass hands to the int32
+rescore is checked."""
 import numpy as np
 import pytest
 
@@ -108,3 +110,26 @@ def test_near_copies_tenth_scale(c5_tenth, matrix, go, ge):
         assert np.array_equal(forced, got)
     d.close()
     e.close()
+
+
+def test_overflow_stress_full_size():
+    """2M sequences (4.3G residues, 10% of 3,001-35,000 residues, 20% near-copies
+    of the two queries), both queries in one launch with BLOSUM50 13/2: the
+    copies of the 5478-residue query that reach the int16 pass's exact limit
+    are rescored in int32; all of them, the longest sequences and a random
+    sample equal the oracle."""
+    db, qs, planted = G.c5_db()
+    e = Engine("blosum50", 13, 2)
+    d = e.load_db(db)
+    got, st = e.search_batch(d, qs)
+    d.close()
+    e.close()
+    assert st["cells"] == db.total * sum(len(q) for q in qs)
+    rescored = [np.flatnonzero(got[i] >= 31744) for i in range(len(qs))]
+    assert st["n_overflow"] == sum(r.size for r in rescored) > 0
+    rng = np.random.Generator(np.random.PCG64(7005))
+    base = sample_idx(db, rng, n_random=300, n_longest=10)
+    for i, q in enumerate(qs):
+        assert got[i][planted].min() > 0
+        idx = np.unique(np.concatenate([base, rescored[i], planted[:20]]))
+        check_sample(q, db, got[i], idx, A.matrix("blosum50"), 13, 2)
