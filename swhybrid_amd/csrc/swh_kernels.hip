@@ -11,16 +11,18 @@
 // Scan kernel (int16): every lane carries TWO database sequences, one in each
 // 16-bit half of a VGPR, and all arithmetic is packed and saturating
 // (v_pk_add_i16 clamp / v_pk_sub_i16 clamp / v_pk_max_i16, and for the
-// three-way maxima v_pk_minimum3_f16 on the same bit patterns, see vmax3).  Values are biased
-// so that 0x8000 encodes score 0: the saturating floor at -32768 is the
+// three-way maxima v_pk_minimum3_f16 on the same bit patterns, see vmax3).
+// Values are biased so that 0x8000 encodes score 0: the saturating floor at -32768 is the
 // local-alignment zero floor, exactly as the reference's 0x80 / 0x8000 bias
 // (src/AVXKernel.cpp:443, :903).  A lane keeps a strip of W query columns in
-// registers (H, E and H-(open+ext) per column) and walks down its sequences'
-// rows; the H/F values on the strip's right edge go to a per-wave border array
-// in HBM and feed the next strip.  The query profile (24 x qlen int16) is
-// staged in LDS per 1344-column segment; each row reads the profile rows of the
-// lane's two residues with 16-byte LDS loads and interleaves them with one
-// v_perm_b32 per column.
+// registers (H and E per column) and walks down its sequences' rows; the H/F
+// values on the strip's right edge go to a per-wave border array in HBM and
+// feed the next strip.  The profile comes from LDS: either a residue-pair
+// table built per strip (a row reads its pair's W dwords directly), or the
+// query profile (24 x qlen int16) staged per 1344-column segment, whose two
+// residue rows a row interleaves with one v_perm_b32 per column.  Sequences
+// too long for one lane are swept by groups of lanes (group_item), and a
+// query's last strip is as narrow as its length rounded up to 8 allows.
 //
 // A sequence whose int16 score reaches the overflow threshold (default 31744,
 // below which no add can have saturated and the three-way max orders) is
@@ -161,14 +163,6 @@ __device__ __forceinline__ void interleave_profile(s2 (&P)[W], const RawProfile<
         P[8 * k + 6] = as_s2(__builtin_amdgcn_perm(vb.w, va.w, 0x05040100u));
         P[8 * k + 7] = as_s2(__builtin_amdgcn_perm(vb.w, va.w, 0x07060302u));
     }
-}
-
-template <int W>
-__device__ __forceinline__ void pair_profile(s2 (&P)[W], const short* lds, int ps, int a, int b,
-                                             int jj0) {
-    RawProfile<W> rp;
-    load_profile<W>(rp, lds, ps, a, b, jj0);
-    interleave_profile<W>(P, rp);
 }
 
 // Pair table: for the W columns of one strip, the interleaved profile pairs
