@@ -735,7 +735,7 @@ int enqueue_tasks(swh_ctx* c, swh_db* db, const Task* t, int nt, const Shape& sh
 // scores_host gets nq rows of nseq.
 // The layout a launch of nq queries runs on: the small-database pair layout
 // when the lane layout would give fewer lane items than workgroup slots.
-static swh_db* pick_layout(const swh_ctx* c, swh_db* db, int nq) {
+swh_db* pick_layout(const swh_ctx* c, swh_db* db, int nq) {
     if (!db->alt) return db;
     return enough_lane_items(c, (uint64_t)(db->n_blocks - db->n_pairs), nq) ? db : db->alt;
 }

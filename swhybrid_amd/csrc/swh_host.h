@@ -214,6 +214,9 @@ Shape choose_shape(const swh_ctx* c, const int* qlens, int nq, const swh_search_
 int group_lanes_for(int qlen, int W);
 // Cost-weighted lane columns of a query at strip width W.
 double lane_cols(int qlen, int W);
+// The layout of db (itself or its small-database second layout) that a launch
+// of nq queries over the whole database runs on.
+swh_db* pick_layout(const swh_ctx* c, swh_db* db, int nq);
 
 // What one launch sequence owns: its stream, descriptor staging and strip
 // borders.  Two of them let two launch sequences be in flight at once.

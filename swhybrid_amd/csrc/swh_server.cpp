@@ -132,6 +132,9 @@ bool run_round(swh_server* s) {
     std::vector<Task> tasks(batch.size());
     std::vector<int> qlens(batch.size());
     const size_t nseq = db->nseq;
+    // whole-database rounds of a small database run on the layout that suits
+    // the round's request count (slices are block ranges of the first layout)
+    if (s->opts.slices == 1) db = pick_layout(c, db, (int)batch.size());
     for (size_t i = 0; i < batch.size(); ++i) {
         Request& r = *batch[i];
         Task& T = tasks[i];
@@ -139,10 +142,17 @@ bool run_round(swh_server* s) {
         T.qlen = (int)r.query.size();
         T.scores = (int32_t*)s->scores.p + (size_t)r.slot * nseq;
         T.ovf = (uint32_t*)s->ovf.p + (size_t)r.slot * s->max_slice_seqs;
-        T.pair_begin = s->pair_cut[r.next_slice];
-        T.pair_end = s->pair_cut[r.next_slice + 1];
-        T.lane_begin = s->lane_cut[r.next_slice];
-        T.lane_end = s->lane_cut[r.next_slice + 1];
+        if (s->opts.slices == 1) {
+            T.pair_begin = 0;
+            T.pair_end = db->n_pairs;
+            T.lane_begin = db->n_pairs;
+            T.lane_end = db->n_blocks;
+        } else {
+            T.pair_begin = s->pair_cut[r.next_slice];
+            T.pair_end = s->pair_cut[r.next_slice + 1];
+            T.lane_begin = s->lane_cut[r.next_slice];
+            T.lane_end = s->lane_cut[r.next_slice + 1];
+        }
         qlens[i] = T.qlen;
     }
     const Shape sh = choose_shape(c, qlens.data(), (int)qlens.size(), s->opts.search);
@@ -282,6 +292,11 @@ int swh_server_create(swh_ctx* c, swh_db* db, const swh_server_opts* opts, swh_s
                          (size_t)(s->lane_cut[i + 1] - s->lane_cut[i]) * kLaneSeqs;
         s->max_slice_seqs = std::max(s->max_slice_seqs, n);
     }
+    if (o.slices == 1 && db->alt)   // a round may run on the second layout
+        s->max_slice_seqs = std::max<size_t>(
+            s->max_slice_seqs,
+            (size_t)db->alt->n_pairs * 2 +
+                (size_t)(db->alt->n_blocks - db->alt->n_pairs) * kLaneSeqs);
     s->max_slice_seqs = std::max<size_t>(s->max_slice_seqs, 1);
     // slots: as many as the budget holds for the longest admissible query
     const size_t per_slot = request_bytes(s.get(), o.max_qlen);
