@@ -441,12 +441,16 @@ static int tune_bucket(const swh_ctx* c, int qlen) {
 }
 
 // Lanes per group for one query: the caller's, else the tuned value of its
-// bucket when that was measured at this group width, else 0 (group_eff).
-static int tuned_group_lanes(const swh_ctx* c, int qlen, int Wg, const swh_search_opts& o) {
-    if (o.group_lanes > 0) return o.group_lanes;
+// bucket when that was measured at this group width (measured at the bucket's
+// longest query, so at most one lane per strip of a shorter one), else the
+// fewest passes (group_lanes_for).
+static int group_lanes(const swh_ctx* c, int qlen, int Wg, const swh_search_opts& o) {
+    if (o.group_lanes > 0) return std::min(o.group_lanes, kWave);
+    const int fewest_passes = group_lanes_for(qlen, Wg);
     const int b = tune_bucket(c, qlen);
-    if (b >= 0 && c->tune[b].group_width == Wg) return c->tune[b].group_lanes;
-    return 0;
+    if (b >= 0 && c->tune[b].group_width == Wg && c->tune[b].group_lanes > 0)
+        return std::min({c->tune[b].group_lanes, kWave, fewest_passes});
+    return fewest_passes;
 }
 
 // Lane columns a query costs at strip width W: it is computed to its length
@@ -571,8 +575,7 @@ int enqueue_tasks(swh_ctx* c, swh_db* db, const Task* t, int nt, const Shape& sh
         QueryDesc& Gd = hd[nt + i];
         Gd = base;
         Gd.qpad = ((T.qlen + sh.Wg - 1) / sh.Wg) * sh.Wg;
-        const int req_g = tuned_group_lanes(c, T.qlen, sh.Wg, o);
-        Gd.group = req_g > 0 ? std::min(req_g, kWave) : group_lanes_for(T.qlen, sh.Wg);
+        Gd.group = group_lanes(c, T.qlen, sh.Wg, o);
         Gd.blk_begin = T.pair_begin;
         Gd.blk_end = T.pair_end;
         const int per_item = kWavesPerWG * (kWave / Gd.group);
