@@ -378,20 +378,49 @@ uint64_t range_padded(const swh_db* db, int b0, int b1) {
     return b1 > b0 ? db->blk_padded[b1] - db->blk_padded[b0] : 0;
 }
 
-// Group mode shape for one query: lanes per group G at column width W.  A
-// group covers the query in as few passes as it can (G = strips, at most 64):
-// a pair's item lasts passes x its rows, and the longest pairs' items must
-// not outlast the launch.
-int group_lanes_for(int qlen, int W) { return std::min((qlen + W - 1) / W, kWave); }
-
-// Useful share of a wave's lane-columns in group mode at width W with
-// G = group_lanes_for: strips / (passes x G) of a pass, (64 / G) x G / 64 of
-// the wave.
-static double group_lane_use(int qlen, int W) {
+// Group mode shape for one query: lanes per group G at column width W, in at
+// most max_passes passes over the query.  G maximises the useful share of a
+// wave's lane-steps: strips / (passes x G) of a pass, times (64 / G) x G / 64
+// of the wave; ties go to fewer passes.  With max_passes = 1 that is the
+// fewest passes (G = strips, at most 64).
+int group_lanes_for(int qlen, int W, int max_passes) {
     const int strips = (qlen + W - 1) / W;
-    const int G = group_lanes_for(qlen, W);
+    const int g_hi = std::min(strips, kWave);
+    const int g_lo = std::max(1, std::min((strips + max_passes - 1) / max_passes, g_hi));
+    int best = g_hi;
+    double best_use = -1.0;
+    for (int G = g_hi; G >= g_lo; --G) {
+        const int passes = (strips + G - 1) / G;
+        if (passes > max_passes) continue;
+        const double use =
+            (double)strips / ((double)passes * G) * (double)((kWave / G) * G) / kWave;
+        if (use > best_use + 1e-9) {
+            best_use = use;
+            best = G;
+        }
+    }
+    return best;
+}
+
+static double group_lane_use(int qlen, int W, int max_passes) {
+    const int strips = (qlen + W - 1) / W;
+    const int G = group_lanes_for(qlen, W, max_passes);
     const int passes = (strips + G - 1) / G;
     return (double)strips / ((double)passes * G) * (double)((kWave / G) * G) / kWave;
+}
+
+// Passes a group item may make over its query at width W: a pair's item runs
+// passes x (rows + G) steps of W columns on one wave, and the longest pair's
+// must stay within 0.3 of the launch's cells per lane (launch cells / (2
+// sequences x 512 lanes x CUs)), since longest-first ordering starts it
+// first.  On the 5M-sequence database (pairs up to 35,000 rows) that allows
+// one pass at query length 464 and two at 1000 (W = 24); on the 2M-sequence
+// overflow-stress database, 11 at 5478.
+static int group_pass_budget(const swh_ctx* c, uint64_t launch_cells, uint32_t rows_max, int W) {
+    if (rows_max == 0) return kWave;
+    const double per_lane = (double)launch_cells / (2.0 * 512.0 * std::max(c->num_cus, 1));
+    const double item = ((double)rows_max + kWave) * W;
+    return std::max(1, std::min(kWave, (int)(0.3 * per_lane / item)));
 }
 
 // Profile row stride of a query: a multiple of 96 covers every strip width
@@ -443,14 +472,14 @@ static int tune_bucket(const swh_ctx* c, int qlen) {
 // Lanes per group for one query: the caller's, else the tuned value of its
 // bucket when that was measured at this group width (measured at the bucket's
 // longest query, so at most one lane per strip of a shorter one), else the
-// fewest passes (group_lanes_for).
-static int group_lanes(const swh_ctx* c, int qlen, int Wg, const swh_search_opts& o) {
+// best lane use within the launch's pass budget (group_lanes_for).
+static int group_lanes(const swh_ctx* c, int qlen, int Wg, const swh_search_opts& o,
+                       int max_passes) {
     if (o.group_lanes > 0) return std::min(o.group_lanes, kWave);
-    const int fewest_passes = group_lanes_for(qlen, Wg);
     const int b = tune_bucket(c, qlen);
     if (b >= 0 && c->tune[b].group_width == Wg && c->tune[b].group_lanes > 0)
-        return std::min({c->tune[b].group_lanes, kWave, fewest_passes});
-    return fewest_passes;
+        return std::min(c->tune[b].group_lanes, group_lanes_for(qlen, Wg, 1));
+    return group_lanes_for(qlen, Wg, max_passes);
 }
 
 // Lane columns a query costs at strip width W: it is computed to its length
@@ -488,7 +517,8 @@ static double builtin_cost(int k) {
 // width of the bucket holding most of the batch's columns, else 16 or 24 by
 // lane use (24 is instantiated with 24-column lanes only; 8 columns per lane
 // lose 2% on the 5M-sequence database at query length 144).
-Shape choose_shape(const swh_ctx* c, const int* qlens, int nq, const swh_search_opts& o) {
+Shape choose_shape(const swh_ctx* c, const int* qlens, int nq, const swh_search_opts& o,
+                   const swh_db* db, uint64_t db_residues) {
     Shape sh{16, o.group_width, true};
     double best = 1e300;
     for (int k = 0; k < SWH_TUNE_LANE_CONFIGS; ++k) {
@@ -518,10 +548,14 @@ Shape choose_shape(const swh_ctx* c, const int* qlens, int nq, const swh_search_
         // 10% faster step (measured at equal lane use), covers more columns
         sh.Wg = 16;
         if (sh.Wl == 24) {
+            uint64_t cells = 0;
+            for (int i = 0; i < nq; ++i) cells += (uint64_t)qlens[i] * db_residues;
+            const int p16 = group_pass_budget(c, cells, db->max_rows_pair, 16);
+            const int p24 = group_pass_budget(c, cells, db->max_rows_pair, 24);
             double u16 = 0, u24 = 0;
             for (int i = 0; i < nq; ++i) {
-                u16 += qlens[i] * group_lane_use(qlens[i], 16);
-                u24 += qlens[i] * group_lane_use(qlens[i], 24) * 1.10;
+                u16 += qlens[i] * group_lane_use(qlens[i], 16, p16);
+                u24 += qlens[i] * group_lane_use(qlens[i], 24, p24) * 1.10;
             }
             if (u24 > u16) sh.Wg = 24;
         }
@@ -549,6 +583,11 @@ int enqueue_tasks(swh_ctx* c, swh_db* db, const Task* t, int nt, const Shape& sh
                                                : exact_thr;
     const int gap_oe = c->gap_open + c->gap_extend;
     std::vector<QueryDesc> hd(3 * (size_t)nt);
+    uint64_t launch_cells = 0;
+    for (int i = 0; i < nt; ++i)
+        launch_cells += (uint64_t)t[i].qlen * (range_residues(db, t[i].pair_begin, t[i].pair_end) +
+                                               range_residues(db, t[i].lane_begin, t[i].lane_end));
+    const int max_passes = group_pass_budget(c, launch_cells, db->max_rows_pair, sh.Wg);
     int items_l = 0, items_g = 0, max_qpad_l = 0, max_pass_g = 0, max_gpw = 1, max_pass_r = 0;
     int max_gpw_r = 0;   // lane groups per wave of the multi-pass rescores (0: none)
     bool lane_border = false, group_border = false;
@@ -575,7 +614,7 @@ int enqueue_tasks(swh_ctx* c, swh_db* db, const Task* t, int nt, const Shape& sh
         QueryDesc& Gd = hd[nt + i];
         Gd = base;
         Gd.qpad = ((T.qlen + sh.Wg - 1) / sh.Wg) * sh.Wg;
-        Gd.group = group_lanes(c, T.qlen, sh.Wg, o);
+        Gd.group = group_lanes(c, T.qlen, sh.Wg, o, max_passes);
         Gd.blk_begin = T.pair_begin;
         Gd.blk_end = T.pair_end;
         const int per_item = kWavesPerWG * (kWave / Gd.group);
@@ -759,7 +798,7 @@ int run_queries(swh_ctx* c, swh_db* db_in, const uint8_t* const* qs, const int* 
         if (stats) *stats = st;
         return SWH_OK;
     }
-    const Shape sh = choose_shape(c, qlens, nq, o);
+    const Shape sh = choose_shape(c, qlens, nq, o, db, db->residues);
     std::vector<size_t> prof_off(nq);
     size_t prof_elems = 0;
     for (int i = 0; i < nq; ++i) {

@@ -209,9 +209,12 @@ uint64_t range_residues(const swh_db* db, int b0, int b1);
 uint64_t range_padded(const swh_db* db, int b0, int b1);
 int check_opts(const swh_search_opts& o);
 int check_query(const uint8_t* q, int qlen, int i);
-Shape choose_shape(const swh_ctx* c, const int* qlens, int nq, const swh_search_opts& o);
-// Lanes per group in group mode: the fewest passes over the query.
-int group_lanes_for(int qlen, int W);
+// db_residues: the residues each query of the launch searches in db.
+Shape choose_shape(const swh_ctx* c, const int* qlens, int nq, const swh_search_opts& o,
+                   const swh_db* db, uint64_t db_residues);
+// Lanes per group in group mode within max_passes passes over the query
+// (1: the fewest passes).
+int group_lanes_for(int qlen, int W, int max_passes);
 // Cost-weighted lane columns of a query at strip width W.
 double lane_cols(int qlen, int W);
 // The layout of db (itself or its small-database second layout) that a launch

@@ -155,7 +155,10 @@ bool run_round(swh_server* s) {
         }
         qlens[i] = T.qlen;
     }
-    const Shape sh = choose_shape(c, qlens.data(), (int)qlens.size(), s->opts.search);
+    const Shape sh = choose_shape(
+        c, qlens.data(), (int)qlens.size(), s->opts.search, db,
+        range_residues(db, tasks[0].pair_begin, tasks[0].pair_end) +
+            range_residues(db, tasks[0].lane_begin, tasks[0].lane_end));
     swh_stats st{};
     int rc = enqueue_tasks(c, db, tasks.data(), (int)tasks.size(), sh, s->opts.search,
                            (uint32_t*)s->counters.p, ctx_res(c), &st);
