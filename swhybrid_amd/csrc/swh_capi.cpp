@@ -382,11 +382,14 @@ uint64_t range_padded(const swh_db* db, int b0, int b1) {
 // most max_passes passes over the query.  G maximises the useful share of a
 // wave's lane-steps: strips / (passes x G) of a pass, times (64 / G) x G / 64
 // of the wave; ties go to fewer passes.  With max_passes = 1 that is the
-// fewest passes (G = strips, at most 64).
-int group_lanes_for(int qlen, int W, int max_passes) {
+// fewest passes (G = strips, at most 64).  G is at least min_lanes (when the
+// query has that many strips), which bounds the groups per wave whose
+// borders a multi-pass item keeps (group_min_lanes).
+int group_lanes_for(int qlen, int W, int max_passes, int min_lanes) {
     const int strips = (qlen + W - 1) / W;
     const int g_hi = std::min(strips, kWave);
-    const int g_lo = std::max(1, std::min((strips + max_passes - 1) / max_passes, g_hi));
+    const int g_lo = std::max({1, std::min((strips + max_passes - 1) / max_passes, g_hi),
+                               std::min(min_lanes, g_hi)});
     int best = g_hi;
     double best_use = -1.0;
     for (int G = g_hi; G >= g_lo; --G) {
@@ -402,9 +405,9 @@ int group_lanes_for(int qlen, int W, int max_passes) {
     return best;
 }
 
-static double group_lane_use(int qlen, int W, int max_passes) {
+static double group_lane_use(int qlen, int W, int max_passes, int min_lanes) {
     const int strips = (qlen + W - 1) / W;
-    const int G = group_lanes_for(qlen, W, max_passes);
+    const int G = group_lanes_for(qlen, W, max_passes, min_lanes);
     const int passes = (strips + G - 1) / G;
     return (double)strips / ((double)passes * G) * (double)((kWave / G) * G) / kWave;
 }
@@ -416,6 +419,17 @@ static double group_lane_use(int qlen, int W, int max_passes) {
 // first.  On the 5M-sequence database (pairs up to 35,000 rows) that allows
 // one pass at query length 464 and two at 1000 (W = 24); on the 2M-sequence
 // overflow-stress database, 11 at 5478.
+// Fewest lanes per group such that every wave slot of a full grid (at most 3
+// workgroups of 4 waves per CU) can keep a border of its longest pair for
+// each of its 64 / G groups within the context's border budget, so that more
+// passes never shrink the grid.
+static int group_min_lanes(const swh_ctx* c, uint32_t rows_max) {
+    const uint64_t per_group = (uint64_t)std::max(c->num_cus, 1) * 3 * kWavesPerWG *
+                               sizeof(uint2) * std::max<uint32_t>(rows_max, 1);
+    const uint64_t m = std::max<uint64_t>(1, std::min<uint64_t>(kWave, c->border_budget / per_group));
+    return (int)(kWave / (m + 1)) + 1;
+}
+
 static int group_pass_budget(const swh_ctx* c, uint64_t launch_cells, uint32_t rows_max, int W) {
     if (rows_max == 0) return kWave;
     const double per_lane = (double)launch_cells / (2.0 * 512.0 * std::max(c->num_cus, 1));
@@ -474,12 +488,12 @@ static int tune_bucket(const swh_ctx* c, int qlen) {
 // longest query, so at most one lane per strip of a shorter one), else the
 // best lane use within the launch's pass budget (group_lanes_for).
 static int group_lanes(const swh_ctx* c, int qlen, int Wg, const swh_search_opts& o,
-                       int max_passes) {
+                       int max_passes, int min_lanes) {
     if (o.group_lanes > 0) return std::min(o.group_lanes, kWave);
     const int b = tune_bucket(c, qlen);
     if (b >= 0 && c->tune[b].group_width == Wg && c->tune[b].group_lanes > 0)
-        return std::min(c->tune[b].group_lanes, group_lanes_for(qlen, Wg, 1));
-    return group_lanes_for(qlen, Wg, max_passes);
+        return std::min(c->tune[b].group_lanes, group_lanes_for(qlen, Wg, 1, 1));
+    return group_lanes_for(qlen, Wg, max_passes, min_lanes);
 }
 
 // Lane columns a query costs at strip width W: it is computed to its length
@@ -552,10 +566,11 @@ Shape choose_shape(const swh_ctx* c, const int* qlens, int nq, const swh_search_
             for (int i = 0; i < nq; ++i) cells += (uint64_t)qlens[i] * db_residues;
             const int p16 = group_pass_budget(c, cells, db->max_rows_pair, 16);
             const int p24 = group_pass_budget(c, cells, db->max_rows_pair, 24);
+            const int gmin = group_min_lanes(c, db->max_rows_pair);
             double u16 = 0, u24 = 0;
             for (int i = 0; i < nq; ++i) {
-                u16 += qlens[i] * group_lane_use(qlens[i], 16, p16);
-                u24 += qlens[i] * group_lane_use(qlens[i], 24, p24) * 1.10;
+                u16 += qlens[i] * group_lane_use(qlens[i], 16, p16, gmin);
+                u24 += qlens[i] * group_lane_use(qlens[i], 24, p24, gmin) * 1.10;
             }
             if (u24 > u16) sh.Wg = 24;
         }
@@ -588,6 +603,7 @@ int enqueue_tasks(swh_ctx* c, swh_db* db, const Task* t, int nt, const Shape& sh
         launch_cells += (uint64_t)t[i].qlen * (range_residues(db, t[i].pair_begin, t[i].pair_end) +
                                                range_residues(db, t[i].lane_begin, t[i].lane_end));
     const int max_passes = group_pass_budget(c, launch_cells, db->max_rows_pair, sh.Wg);
+    const int min_lanes = group_min_lanes(c, db->max_rows_pair);
     int items_l = 0, items_g = 0, max_qpad_l = 0, max_pass_g = 0, max_gpw = 1, max_pass_r = 0;
     int max_gpw_r = 0;   // lane groups per wave of the multi-pass rescores (0: none)
     bool lane_border = false, group_border = false;
@@ -614,7 +630,7 @@ int enqueue_tasks(swh_ctx* c, swh_db* db, const Task* t, int nt, const Shape& sh
         QueryDesc& Gd = hd[nt + i];
         Gd = base;
         Gd.qpad = ((T.qlen + sh.Wg - 1) / sh.Wg) * sh.Wg;
-        Gd.group = group_lanes(c, T.qlen, sh.Wg, o, max_passes);
+        Gd.group = group_lanes(c, T.qlen, sh.Wg, o, max_passes, min_lanes);
         Gd.blk_begin = T.pair_begin;
         Gd.blk_end = T.pair_end;
         const int per_item = kWavesPerWG * (kWave / Gd.group);

@@ -213,8 +213,8 @@ int check_query(const uint8_t* q, int qlen, int i);
 Shape choose_shape(const swh_ctx* c, const int* qlens, int nq, const swh_search_opts& o,
                    const swh_db* db, uint64_t db_residues);
 // Lanes per group in group mode within max_passes passes over the query
-// (1: the fewest passes).
-int group_lanes_for(int qlen, int W, int max_passes);
+// (1: the fewest passes), at least min_lanes.
+int group_lanes_for(int qlen, int W, int max_passes, int min_lanes);
 // Cost-weighted lane columns of a query at strip width W.
 double lane_cols(int qlen, int W);
 // The layout of db (itself or its small-database second layout) that a launch

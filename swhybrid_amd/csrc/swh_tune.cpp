@@ -238,7 +238,7 @@ int swh_autotune(swh_ctx* c, const uint8_t* residues, const uint64_t* offsets,
         if (E.strip_width == 24) widths.push_back(24);
         for (int wg : widths) {
             const int strips = (L + wg - 1) / wg;
-            std::vector<int> cand = {group_lanes_for(L, wg, 1)};
+            std::vector<int> cand = {group_lanes_for(L, wg, 1, 1)};
             for (int g : {8, 16, 32})
                 if (g < strips) cand.push_back(g);
             std::sort(cand.begin(), cand.end());
