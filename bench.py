@@ -412,6 +412,9 @@ def main():
             eng.search(d, q)
         log(f"warmup step {w + 1}/{args.warmup} done")
 
+    # the caller keeps one page-locked score vector: the device writes it
+    # directly, and the top-k is taken from it before the next call
+    out = eng.pinned_scores(db.n)
     cells = 0
     kernel_ms = 0.0
     per_query = {}
@@ -419,7 +422,7 @@ def main():
     t_start = time.perf_counter()
     for s in range(args.steps):
         for q in queries:
-            scores, st = eng.search(d, q)
+            scores, st = eng.search(d, q, out=out)
             tops[len(q)] = topk(scores, TOP_K)
             cells += st["cells"]
             kernel_ms += st["kernel_ms"]

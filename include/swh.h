@@ -123,7 +123,9 @@ int swh_db_destroy(swh_db* db);
 int swh_db_info(const swh_db* db, uint64_t* nseq, uint64_t* residues, uint64_t* device_bytes);
 
 /* One query, one search: int16 scan + on-device int32 rescore; scores[i] for
- * every database sequence in caller order.
+ * every database sequence in caller order.  A page-locked scores buffer
+ * (swh_pinned_alloc) is written by the device copy itself, with no host copy
+ * after it.
  * Replaces kernelLaunch + bindQueryPrf (src/CUDAKernel.h:9-22,
  * src/CUDAKernel.cu:194-226) and the score copy-back of CUDAWorker::search
  * (src/CUDAWorker.cu:224-246). */

@@ -242,3 +242,25 @@ def check(rc: int):
 
 def ptr(a: np.ndarray):
     return a.ctypes.data_as(C.c_void_p)
+
+
+class _Pinned:
+    """Page-locked host memory from swh_pinned_alloc, exposed to numpy through
+    the array interface and freed with the last array that uses it."""
+
+    def __init__(self, n, dtype):
+        dt = np.dtype(dtype)
+        self.addr = C.c_void_p()
+        check(lib().swh_pinned_alloc(max(1, n) * dt.itemsize, C.byref(self.addr)))
+        self.__array_interface__ = {"shape": (n,), "typestr": dt.str,
+                                    "data": (self.addr.value, False), "version": 3}
+
+    def __del__(self):
+        if getattr(self, "addr", None) is not None and self.addr.value:
+            lib().swh_pinned_free(self.addr)
+            self.addr = None
+
+
+def pinned_array(n, dtype=np.int32) -> np.ndarray:
+    """A 1-d array of n elements in page-locked host memory."""
+    return np.asarray(_Pinned(n, dtype))

@@ -789,8 +789,21 @@ int enqueue_tasks(swh_ctx* c, swh_db* db, const Task* t, int nt, const Shape& sh
     return SWH_OK;
 }
 
+// Whether p is page-locked host memory (hipHostMalloc, swh_pinned_alloc) that
+// a device copy can write directly.
+static bool is_pinned_host(const void* p) {
+    hipPointerAttribute_t a{};
+    if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+        (void)hipGetLastError();   // pageable memory: not an error of this call
+        return false;
+    }
+    return a.type == hipMemoryTypeHost;
+}
+
 // Search every query over the whole database in one launch sequence.
-// scores_host gets nq rows of nseq.
+// scores_host gets nq rows of nseq: copied there straight from the device
+// when it is page-locked, else through the context's pinned staging (up to 64
+// MB) and a host copy.
 // The layout a launch of nq queries runs on: the small-database pair layout
 // when the lane layout would give fewer lane items than workgroup slots.
 swh_db* pick_layout(const swh_ctx* c, swh_db* db, int nq) {
@@ -829,7 +842,8 @@ int run_queries(swh_ctx* c, swh_db* db_in, const uint8_t* const* qs, const int* 
     HIP_TRY(c->ovf.reserve((size_t)nq * nseq * sizeof(uint32_t)));
     HIP_TRY(c->counters.reserve((16 + (size_t)nq) * sizeof(uint32_t)));
     HIP_TRY(c->h_ctr.reserve((16 + (size_t)nq) * sizeof(uint32_t)));
-    const bool pinned_out = score_bytes <= (size_t(64) << 20);
+    const bool direct_out = is_pinned_host(scores_host);
+    const bool pinned_out = !direct_out && score_bytes <= (size_t(64) << 20);
     if (pinned_out) HIP_TRY(c->h_scores.reserve(score_bytes));
     int16_t* hp = (int16_t*)c->h_prof.p;
     std::vector<Task> tasks(nq);
@@ -852,9 +866,9 @@ int run_queries(swh_ctx* c, swh_db* db_in, const uint8_t* const* qs, const int* 
         if (r2) return r2;
         HIP_TRY(hipMemcpyAsync(c->h_ctr.p, ctr, (16 + (size_t)nq) * sizeof(uint32_t),
                                hipMemcpyDeviceToHost, c->stream));
-        if (pinned_out)
-            HIP_TRY(hipMemcpyAsync(c->h_scores.p, c->scores.p, score_bytes,
-                                   hipMemcpyDeviceToHost, c->stream));
+        if (pinned_out || direct_out)
+            HIP_TRY(hipMemcpyAsync(direct_out ? (void*)scores_host : c->h_scores.p, c->scores.p,
+                                   score_bytes, hipMemcpyDeviceToHost, c->stream));
         return SWH_OK;
     };
     std::vector<long long> key;
@@ -862,7 +876,8 @@ int run_queries(swh_ctx* c, swh_db* db_in, const uint8_t* const* qs, const int* 
         // the database by id: a new one may reuse a destroyed one's address
         key = {(long long)db->uid, nq, sh.Wl, sh.Wg, sh.pair, (long long)c->tune_gen,
                o.group_lanes, o.overflow_threshold,
-               pinned_out, (long long)(uintptr_t)c->prof.p, (long long)(uintptr_t)c->scores.p,
+               pinned_out, (long long)(uintptr_t)(direct_out ? scores_host : nullptr),
+               (long long)(uintptr_t)c->prof.p, (long long)(uintptr_t)c->scores.p,
                (long long)(uintptr_t)c->ovf.p, (long long)(uintptr_t)c->counters.p,
                (long long)(uintptr_t)c->h_prof.p, (long long)(uintptr_t)c->h_ctr.p,
                (long long)(uintptr_t)c->h_scores.p};
@@ -918,7 +933,8 @@ int run_queries(swh_ctx* c, swh_db* db_in, const uint8_t* const* qs, const int* 
         c->graph.desc_bytes.assign(hb, hb + c->desc_used);
     }
     if (pinned_out) std::memcpy(scores_host, c->h_scores.p, score_bytes);
-    else HIP_TRY(hipMemcpy(scores_host, c->scores.p, score_bytes, hipMemcpyDeviceToHost));
+    else if (!direct_out)
+        HIP_TRY(hipMemcpy(scores_host, c->scores.p, score_bytes, hipMemcpyDeviceToHost));
     const uint32_t* hc = (const uint32_t*)c->h_ctr.p;
     float ms = 0;
     HIP_TRY(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));

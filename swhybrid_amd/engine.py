@@ -102,20 +102,33 @@ class Engine:
                                       db.n, C.byref(o), C.byref(h)))
         return Database(self, h, db.n, db.total)
 
-    def search(self, db: Database, query, **opts):
+    def pinned_scores(self, n, nq=1):
+        """A page-locked int32 array of nq x n (1-d for nq = 1) for the out=
+        argument of search / search_batch: the device copies scores straight
+        into it.  Freed when the array is."""
+        return N.pinned_array(n * nq, np.int32).reshape((nq, n) if nq > 1 else (n,))
+
+    def search(self, db: Database, query, out=None, **opts):
         q = _as_query(query)
-        out = np.empty(db.n, dtype=np.int32)
+        if out is None:
+            out = np.empty(db.n, dtype=np.int32)
+        elif out.dtype != np.int32 or out.size < db.n or not out.flags.c_contiguous:
+            raise ValueError("out must be a contiguous int32 array of at least db.n entries")
         st = N.SwhStats()
         o = _opts(**opts)
         N.check(N.lib().swh_search(self.handle, db.handle, N.ptr(q), len(q), C.byref(o),
                                    N.ptr(out), C.byref(st)))
         return out, st.as_dict()
 
-    def search_batch(self, db: Database, queries, **opts):
+    def search_batch(self, db: Database, queries, out=None, **opts):
         qs = [_as_query(q) for q in queries]
         arr = (C.c_void_p * len(qs))(*[q.ctypes.data for q in qs])
         lens = np.array([len(q) for q in qs], dtype=np.int32)
-        out = np.empty((len(qs), db.n), dtype=np.int32)
+        if out is None:
+            out = np.empty((len(qs), db.n), dtype=np.int32)
+        elif (out.dtype != np.int32 or out.size < len(qs) * db.n
+              or not out.flags.c_contiguous):
+            raise ValueError("out must be a contiguous int32 array of at least nq x db.n")
         st = N.SwhStats()
         o = _opts(**opts)
         N.check(N.lib().swh_search_batch(self.handle, db.handle, arr, N.ptr(lens), len(qs),

@@ -372,6 +372,30 @@ def test_small_database_second_layout(eng):
     d.close()
 
 
+def test_page_locked_output(eng):
+    """Scores copied by the device straight into a page-locked caller buffer
+    equal those through the staging copy, eager and replayed from a graph,
+    single and batched; a reused buffer is overwritten each call."""
+    rng = np.random.Generator(np.random.PCG64(93))
+    db = G.SeqDB.from_lengths(rng, G.lognormal_lengths(rng, 3000, 300, 0.8, 10, 3000))
+    d = eng.load_db(db, group_min_len=1500)
+    qs = [G.random_residues(rng, m) for m in (77, 400, 1300)]
+    pin = eng.pinned_scores(db.n)
+    for g in (0, 1, 1, 1):
+        for q in qs:
+            ref, _ = eng.search(d, q, use_graph=g)
+            got, _ = eng.search(d, q, out=pin, use_graph=g)
+            assert got is pin and np.array_equal(got, ref)
+    many = eng.pinned_scores(db.n, nq=len(qs))
+    ref, _ = eng.search_batch(d, qs)
+    got, _ = eng.search_batch(d, qs, out=many)
+    assert np.array_equal(got, ref)
+    assert np.array_equal(ref[1], O.search_i32(qs[1], db, eng.matrix, eng.gap_open, eng.gap_extend))
+    with pytest.raises(ValueError):
+        eng.search(d, qs[0], out=np.empty(db.n - 1, dtype=np.int32))
+    d.close()
+
+
 def test_group_width_24_needs_24_column_lanes(eng):
     db = long_db(77, n=20)
     d = eng.load_db(db, group_min_len=300)
