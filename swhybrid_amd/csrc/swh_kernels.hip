@@ -257,24 +257,30 @@ __device__ __forceinline__ void lane_strip(const uint2* __restrict__ wds, int qu
     }
     s2 diag0 = Z;
     const uint2 zz = make_uint2(kZero16x2, kZero16x2);
-    uint2 wn = wds[lane];
-    uint2 bn[4];
+    // three register sets of (word, borders): tile k computes from one while
+    // the next tile's set is in flight and tile k + 2's loads are issued
+    // (quads >= 1: a block has at least 4 rows)
+    uint2 wa = wds[lane];
+    uint2 ba[4], bb[4], bc3[4];
+    const int k1 = max(min(1, quads - 1), 0);
+    uint2 wb = wds[(size_t)k1 * kWave + lane], wc3 = wb;
 #pragma unroll
-    for (int r = 0; r < 4; ++r) bn[r] = IN ? bord[r * kWave + lane] : zz;
+    for (int r = 0; r < 4; ++r) {
+        ba[r] = IN ? bord[r * kWave + lane] : zz;
+        bb[r] = IN ? bord[((size_t)k1 * 4 + r) * kWave + lane] : zz;
+        bc3[r] = bb[r];
+    }
     typename Src::Raw rp;
-    src.load(rp, wn.x & 0xff, wn.y & 0xff);
-    for (int k = 0; k < quads; ++k) {
-        const uint2 w = wn;
-        uint2 bc[4];
-#pragma unroll
-        for (int r = 0; r < 4; ++r) bc[r] = bn[r];
-        // prefetch the next tile (the last tile reloads itself: no branch)
-        const int kn = min(k + 1, quads - 1);
-        wn = wds[(size_t)kn * kWave + lane];
+    src.load(rp, wa.x & 0xff, wa.y & 0xff);
+    auto tile = [&](int k, const uint2& w, const uint2 (&b)[4], const uint2& wn, uint2& wf,
+                    uint2 (&bf)[4]) {
+        // prefetch tile k + 2 (the last tiles reload the last one: no branch)
+        const int kf = min(k + 2, quads - 1);
+        wf = wds[(size_t)kf * kWave + lane];
         if (IN) {
-            const uint2* bp = bord + (size_t)kn * 4 * kWave + lane;
+            const uint2* bp = bord + (size_t)kf * 4 * kWave + lane;
 #pragma unroll
-            for (int r = 0; r < 4; ++r) bn[r] = bp[r * kWave];
+            for (int r = 0; r < 4; ++r) bf[r] = bp[r * kWave];
         }
         uint2* bo = bord + (size_t)k * 4 * kWave + lane;
         // the profile of row r+1 is loaded while row r computes: the barrier
@@ -290,8 +296,8 @@ __device__ __forceinline__ void lane_strip(const uint2* __restrict__ wds, int qu
             __builtin_amdgcn_sched_barrier(0);
             s2 P[W];
             src.pairs(P, cur);
-            const s2 hl = as_s2(bc[r].x);
-            const s2 fl = as_s2(bc[r].y);
+            const s2 hl = as_s2(b[r].x);
+            const s2 fl = as_s2(b[r].y);
             s2 fout;
             row_cells<W, s2>(h, e, P, diag0, hl, fl, Q, R, fout);
             S = row_max_into<W, s2>(S, h);
@@ -299,7 +305,17 @@ __device__ __forceinline__ void lane_strip(const uint2* __restrict__ wds, int qu
             diag0 = hl;
             __builtin_amdgcn_sched_barrier(0);
         }
+    };
+    // unrolled three times with the sets rotating roles, so no copies move the
+    // prefetched values (rotating them made every tile wait for its loads)
+    int k = 0;
+    for (; k + 2 < quads; k += 3) {
+        tile(k, wa, ba, wb, wc3, bc3);
+        tile(k + 1, wb, bb, wc3, wa, ba);
+        tile(k + 2, wc3, bc3, wa, wb, bb);
     }
+    if (k < quads) tile(k, wa, ba, wb, wc3, bc3);
+    if (k + 1 < quads) tile(k + 1, wb, bb, wc3, wa, ba);
 }
 
 template <int W, typename Src>
@@ -367,7 +383,7 @@ __device__ __forceinline__ void lane_item(const ScanArgs& A, const QueryDesc& Qd
     const int lane = threadIdx.x & 63;
     const s2 Q = {(short)A.gap_oe, (short)A.gap_oe};
     const s2 R = {(short)A.gap_e, (short)A.gap_e};
-    const int blk = Qd.blk_begin + local * kWavesPerWG + wave;
+    const int blk = __builtin_amdgcn_readfirstlane(Qd.blk_begin + local * kWavesPerWG + wave);
     const bool active = blk < Qd.blk_end;
     BlockDesc bd = {0, 0, 0};
     if (active) bd = A.blocks[blk];
@@ -484,26 +500,28 @@ __device__ __forceinline__ void group_item(const ScanArgs& A, const QueryDesc& Q
         const bool reads_border = has_in && gl == 0 && has_pair;
         const int rows = (int)bd.rows;
         const int last_row = max(rows - 1, 0);
-        uint2 wn[4], bn[4];
+        uint2 wa[4], ba[4], wb[4], bb[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            wn[u] = pw[min(max(u - gl, 0), last_row) >> 2];
-            bn[u] = make_uint2(kZero16x2, kZero16x2);
-            if (reads_border && u < rows) bn[u] = bord[u];
+            wa[u] = pw[min(max(u - gl, 0), last_row) >> 2];
+            ba[u] = make_uint2(kZero16x2, kZero16x2);
+            if (reads_border && u < rows) ba[u] = bord[u];
+            wb[u] = wa[u];
+            bb[u] = ba[u];
         }
         // the profile rows of a step are loaded while the step before it computes
         RawProfile<W> rp;
         {
             const int sh = 8 * ((-gl) & 3);
-            load_profile<W>(rp, lds, A.ps_group, (wn[0].x >> sh) & 0xff, (wn[0].y >> sh) & 0xff,
+            load_profile<W>(rp, lds, A.ps_group, (wa[0].x >> sh) & 0xff, (wa[0].y >> sh) & 0xff,
                             jj0);
         }
-        for (int t0 = 0; t0 < steps; t0 += 4) {
-            uint2 wc[4], bc[4];
+        // 4 steps from (wc, bc) while the next 4 steps' words and borders load
+        // into (wn, bn); unrolled twice with the two sets swapping roles
+        auto quad = [&](int t0, const uint2 (&wc)[4], const uint2 (&bc)[4], uint2 (&wn)[4],
+                        uint2 (&bn)[4]) {
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
-                wc[u] = wn[u];
-                bc[u] = bn[u];
                 wn[u] = pw[min(max(t0 + 4 + u - gl, 0), last_row) >> 2];
                 const int tn = t0 + 4 + u;
                 if (reads_border && tn < rows) bn[u] = bord[tn];
@@ -535,6 +553,11 @@ __device__ __forceinline__ void group_item(const ScanArgs& A, const QueryDesc& Q
                 }
                 __builtin_amdgcn_sched_barrier(0);
             }
+        };
+        for (int t0 = 0; t0 < steps; t0 += 8) {
+            quad(t0, wa, ba, wb, bb);
+            if (t0 + 4 >= steps) break;   // a uniform branch: no tail copy of the code
+            quad(t0 + 4, wb, bb, wa, ba);
         }
     }
     s_red[wave][lane] = bits(S);
@@ -563,9 +586,11 @@ __device__ __forceinline__ bool next_scan_item(const ScanArgs& A, int* s_buf, in
         s_buf[2] = (v.x & kLaneItem) ? 1 : 0;
     }
     __syncthreads();
-    local = s_buf[0];
-    q = s_buf[1];
-    is_group = s_buf[2] == 0;
+    // one value for the whole workgroup: kept in scalar registers, so item
+    // descriptors load through the scalar cache and loops over them are scalar
+    local = __builtin_amdgcn_readfirstlane(s_buf[0]);
+    q = __builtin_amdgcn_readfirstlane(s_buf[1]);
+    is_group = __builtin_amdgcn_readfirstlane(s_buf[2]) == 0;
     return local >= 0;
 }
 
