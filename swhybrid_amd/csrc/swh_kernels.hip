@@ -261,17 +261,19 @@ __device__ __forceinline__ void lane_strip(const uint2* __restrict__ wds, int qu
     // the next tile's set is in flight and tile k + 2's loads are issued
     // (quads >= 1: a block has at least 4 rows)
     uint2 wa = wds[lane];
-    uint2 ba[4], bb[4], bc3[4];
+    uint2 ba[4], bb[4], bc[4];
     const int k1 = max(min(1, quads - 1), 0);
-    uint2 wb = wds[(size_t)k1 * kWave + lane], wc3 = wb;
+    uint2 wb = wds[(size_t)k1 * kWave + lane], wc = wb;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         ba[r] = IN ? bord[r * kWave + lane] : zz;
         bb[r] = IN ? bord[((size_t)k1 * 4 + r) * kWave + lane] : zz;
-        bc3[r] = bb[r];
+        bc[r] = bb[r];
     }
     typename Src::Raw rp;
     src.load(rp, wa.x & 0xff, wa.y & 0xff);
+    // tile k: its word w and borders b, the next tile's word wn (in flight),
+    // and the set (wf, bf) that receives tile k + 2
     auto tile = [&](int k, const uint2& w, const uint2 (&b)[4], const uint2& wn, uint2& wf,
                     uint2 (&bf)[4]) {
         // prefetch tile k + 2 (the last tiles reload the last one: no branch)
@@ -310,12 +312,12 @@ __device__ __forceinline__ void lane_strip(const uint2* __restrict__ wds, int qu
     // prefetched values (rotating them made every tile wait for its loads)
     int k = 0;
     for (; k + 2 < quads; k += 3) {
-        tile(k, wa, ba, wb, wc3, bc3);
-        tile(k + 1, wb, bb, wc3, wa, ba);
-        tile(k + 2, wc3, bc3, wa, wb, bb);
+        tile(k, wa, ba, wb, wc, bc);
+        tile(k + 1, wb, bb, wc, wa, ba);
+        tile(k + 2, wc, bc, wa, wb, bb);
     }
-    if (k < quads) tile(k, wa, ba, wb, wc3, bc3);
-    if (k + 1 < quads) tile(k + 1, wb, bb, wc3, wa, ba);
+    if (k < quads) tile(k, wa, ba, wb, wc, bc);
+    if (k + 1 < quads) tile(k + 1, wb, bb, wc, wa, ba);
 }
 
 template <int W, typename Src>
