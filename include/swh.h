@@ -118,6 +118,11 @@ int swh_ctx_set_scoring(swh_ctx* ctx, const int32_t* matrix, int gap_open, int g
 int swh_db_create(swh_ctx* ctx, const uint8_t* residues, const uint64_t* offsets,
                   const uint32_t* lengths, uint64_t nseq, const swh_db_opts* opts,
                   swh_db** out);
+/* The database's device buffers stay with its context (at most 32 buffers,
+ * 8 GiB) for the next swh_db_create on it and are freed by swh_ctx_destroy:
+ * freeing device memory waits for every search running on the device, and
+ * the worker pipeline builds one chunk's database while the previous chunk is
+ * searched.  No search on db may still be running. */
 int swh_db_destroy(swh_db* db);
 /* nseq, total residues, device bytes held by the packed layout. */
 int swh_db_info(const swh_db* db, uint64_t* nseq, uint64_t* residues, uint64_t* device_bytes);

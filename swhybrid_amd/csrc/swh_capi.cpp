@@ -93,6 +93,9 @@ int swh_ctx_destroy(swh_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->upload) (void)hipStreamSynchronize(c->upload);
     c->graph.release();
+    c->cache.release();
+    c->stage_raw.release();
+    c->stage_soff.release();
     c->border.release();
     c->counters.release();
     c->prof.release();
@@ -149,6 +152,8 @@ static uint32_t default_group_min_len(const swh_ctx* c, const uint32_t* lengths,
 // a 512 MB worker chunk of the 5M-sequence database's longest 500k sequences,
 // 977 lane items, runs 6% faster on the lane layout).
 constexpr uint64_t kSmallDbSeqsPerCu = 1024;
+// Staging buffers up to this size stay with the context between uploads.
+constexpr size_t kStageKeepBytes = size_t(2) << 30;
 constexpr uint32_t kSmallDbGroupMinLen = 128;
 
 // At least one lane item (4 lane blocks) per workgroup slot, 2 slots per CU.
@@ -270,28 +275,40 @@ static int create_layout(swh_ctx* c, const uint8_t* residues, const uint64_t* of
     } else {
         for (uint64_t i = 0; i < nseq; ++i) soff[i] = lengths[i] > 0 ? offsets[i] - lo : 0;
     }
+    // device buffers come from the context's spares where one fits and the
+    // staging buffers are the context's own: no hipFree here (it would wait
+    // for every search running on the device)
     auto up = [&](DevBuf& d, const void* h, size_t bytes) -> hipError_t {
+        hipError_t e = c->cache.take(d, std::max<size_t>(bytes, 16));
+        if (e != hipSuccess || bytes == 0) return e;
+        return hipMemcpy(d.p, h, bytes, hipMemcpyHostToDevice);
+    };
+    std::lock_guard<std::mutex> stage_lock(c->stage_mu);
+    DevBuf& raw = c->stage_raw;
+    DevBuf& d_soff = c->stage_soff;
+    auto stage_up = [&](DevBuf& d, const void* h, size_t bytes) -> hipError_t {
         hipError_t e = d.reserve(std::max<size_t>(bytes, 16));
         if (e != hipSuccess || bytes == 0) return e;
         return hipMemcpy(d.p, h, bytes, hipMemcpyHostToDevice);
     };
-    DevBuf raw, d_soff;
-    hipError_t e = db->words.reserve(std::max<uint64_t>(woff, 1) * 4);
+    hipError_t e = c->cache.take(db->words, std::max<uint64_t>(woff, 1) * 4);
     if (e == hipSuccess && db->n_blocks > 0)
         e = up(db->d_blocks, db->blocks.data(), db->n_blocks * sizeof(BlockDesc));
     if (e == hipSuccess) e = up(db->perm, perm.data(), perm.size() * 4);
     if (e == hipSuccess) e = up(db->sorted_len, slen.data(), slen.size() * 4);
     if (e == hipSuccess) e = up(db->sorted_blk, sblk.data(), sblk.size() * 4);
-    if (e == hipSuccess) e = up(raw, stage, stage_bytes);
-    if (e == hipSuccess) e = up(d_soff, soff.data(), soff.size() * 8);
+    if (e == hipSuccess) e = stage_up(raw, stage, stage_bytes);
+    if (e == hipSuccess) e = stage_up(d_soff, soff.data(), soff.size() * 8);
     if (e == hipSuccess && db->n_blocks > 0)
         e = launch_pack((const BlockDesc*)db->d_blocks.p, db->n_blocks, np,
                         (const uint32_t*)db->perm.p, (const uint32_t*)db->sorted_len.p,
                         (const uint8_t*)raw.p, (const uint64_t*)d_soff.p,
                         (uint32_t*)db->words.p, c->upload);
     if (e == hipSuccess) e = hipStreamSynchronize(c->upload);
-    raw.release();
-    d_soff.release();
+    if (raw.bytes > kStageKeepBytes) {   // a one-off large load: not kept
+        raw.release();
+        d_soff.release();
+    }
     if (e != hipSuccess) {
         swh_db_destroy(db);
         return fail(e == hipErrorOutOfMemory ? SWH_ERR_NOMEM : SWH_ERR_HIP,
@@ -343,12 +360,11 @@ int swh_db_create(swh_ctx* c, const uint8_t* residues, const uint64_t* offsets,
 int swh_db_destroy(swh_db* db) {
     if (!db) return SWH_OK;
     if (db->alt) swh_db_destroy(db->alt);
-    if (db->ctx) (void)hipSetDevice(db->ctx->device);
-    db->words.release();
-    db->d_blocks.release();
-    db->perm.release();
-    db->sorted_len.release();
-    db->sorted_blk.release();
+    if (db->ctx) {   // the buffers go to the context's spares (see BufCache)
+        (void)hipSetDevice(db->ctx->device);
+        for (DevBuf* b : {&db->words, &db->d_blocks, &db->perm, &db->sorted_len, &db->sorted_blk})
+            db->ctx->cache.give(*b);
+    }
     delete db;
     return SWH_OK;
 }

@@ -9,6 +9,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -86,6 +87,71 @@ struct DevBuf {
     }
 };
 
+// Device buffers of destroyed databases, kept by their context for the next
+// database it builds.  hipFree waits for the whole device (an implicit device
+// synchronisation), so freeing them would hold a database build on one
+// thread behind a search running on another: the worker pipeline builds chunk
+// k + 1 while chunk k is searched, and destroys chunk k - 1's database first.
+struct BufCache {
+    static constexpr size_t kMaxBytes = size_t(8) << 30;
+    static constexpr size_t kMaxBuffers = 32;
+    std::mutex mu;
+    std::vector<DevBuf> spare;   // oldest first
+    size_t total = 0;
+    // d gets at least want bytes: its own buffer, the smallest spare one that
+    // is not much larger, or a new allocation.
+    hipError_t take(DevBuf& d, size_t want) {
+        if (want <= d.bytes) return hipSuccess;
+        {
+            std::lock_guard<std::mutex> g(mu);
+            int best = -1;
+            for (size_t i = 0; i < spare.size(); ++i) {
+                const size_t b = spare[i].bytes;
+                if (b >= want && b <= 2 * want + (size_t(64) << 20) &&
+                    (best < 0 || b < spare[(size_t)best].bytes))
+                    best = (int)i;
+            }
+            if (best >= 0) {
+                DevBuf got = spare[(size_t)best];
+                spare.erase(spare.begin() + best);
+                total -= got.bytes;
+                if (d.p) {   // d's own, too small: back to the spares
+                    spare.push_back(d);
+                    total += d.bytes;
+                }
+                d = got;
+                trim();
+                return hipSuccess;
+            }
+        }
+        return d.reserve(want);
+    }
+    void give(DevBuf& d) {
+        if (!d.p) return;
+        std::lock_guard<std::mutex> g(mu);
+        spare.push_back(d);
+        total += d.bytes;
+        d.p = nullptr;
+        d.bytes = 0;
+        trim();
+    }
+    void release() {
+        std::lock_guard<std::mutex> g(mu);
+        for (auto& b : spare) b.release();
+        spare.clear();
+        total = 0;
+    }
+
+  private:
+    void trim() {   // caller holds mu
+        while (!spare.empty() && (spare.size() > kMaxBuffers || total > kMaxBytes)) {
+            total -= spare.front().bytes;
+            spare.front().release();
+            spare.erase(spare.begin());
+        }
+    }
+};
+
 struct PinnedBuf {
     void* p = nullptr;
     size_t bytes = 0;
@@ -122,6 +188,7 @@ inline int check_scoring(const int32_t* matrix, int gap_open, int gap_extend) {
 }  // namespace swhh
 
 
+using swhh::BufCache;
 using swhh::DevBuf;
 using swhh::PinnedBuf;
 using swh::BlockDesc;
@@ -155,6 +222,9 @@ struct swh_ctx {
     hipEvent_t ev[8] = {};
     size_t border_budget = size_t(8) << 30;   // bytes of HBM for strip borders
     GraphCache graph;
+    BufCache cache;                     // buffers of destroyed databases
+    std::mutex stage_mu;                // one database upload at a time
+    DevBuf stage_raw, stage_soff;       // its staging buffers, reused
     std::vector<swh_tune_entry> tune;   // empty: built-in shape choice
     uint64_t tune_gen = 0;              // bumped whenever the table changes
 };
