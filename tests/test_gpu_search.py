@@ -439,3 +439,20 @@ def test_many_queries_one_launch(eng):
         assert np.array_equal(got[i], O.search_i32(qs[i], db, eng.matrix, 10, 2)), f"query {i}"
     assert st["cells"] == sum(db.total * len(q) for q in qs)
     d.close()
+
+
+def test_database_buffers_reused():
+    """A context keeps a destroyed database's device buffers for its next
+    database: a smaller database built into a larger one's buffers (stale
+    words beyond its own blocks), then a larger one again, score exactly."""
+    e = Engine("blosum62", 10, 2)
+    q = G.queries([400], 907)[0]
+    try:
+        for n, seed, hi in ((3000, 908, 3000), (700, 909, 900), (2500, 910, 2500)):
+            db = small_db(n, seed, mean=300, hi=hi)
+            d = e.load_db(db)
+            got, _ = e.search(d, q)
+            assert np.array_equal(got, O.search_i32(q, db, e.matrix, 10, 2)), (n, seed)
+            d.close()
+    finally:
+        e.close()
