@@ -62,6 +62,10 @@ void HIPWorker::packDB() {
     TotalAminoAcidResidue += filledSize;
     offsets_.clear();
     lengths_.clear();
+    size_t n = 0;
+    for (size_t b = 0; b < infoSize; ++b) n += info[b].numSeqs;
+    offsets_.reserve(n);
+    lengths_.reserve(n);
     uint64_t off = 0;
     for (size_t b = 0; b < infoSize; ++b)
         for (unsigned k = 0; k < info[b].numSeqs; ++k) {
