@@ -990,39 +990,60 @@ int swh_search_batch(swh_ctx* c, swh_db* db, const uint8_t* const* queries, cons
 int swh_topk(const int32_t* scores, uint64_t n, int k, int32_t* out_score, int64_t* out_index) {
     if (k < 0 || (k > 0 && (!out_score || !out_index)) || (n > 0 && !scores))
         return fail(SWH_ERR_ARG, "bad top-k arguments");
-    // (score desc, index asc) among scores > 0; a min-heap of the current k best
-    auto worse = [](const std::pair<int32_t, int64_t>& a, const std::pair<int32_t, int64_t>& b) {
+    // (score desc, index asc) among scores > 0: a total order, so the best k
+    // of the union of the best k of each part are the best k of the whole
+    using Entry = std::pair<int32_t, int64_t>;
+    auto worse = [](const Entry& a, const Entry& b) {
         return a.first != b.first ? a.first > b.first : a.second < b.second;
     };
-    std::vector<std::pair<int32_t, int64_t>> heap;
-    heap.reserve(k + 1);
-    // once the heap is full a score must beat its weakest entry; blocks of 64
-    // whose maximum cannot are skipped (a vectorisable max per block)
-    int32_t floor = 0;   // scores <= floor cannot enter
-    constexpr uint64_t kBlock = 64;
-    for (uint64_t b = 0; b < n && k > 0; b += kBlock) {
-        const uint64_t e = std::min(n, b + kBlock);
-        int32_t m = scores[b];
-        for (uint64_t i = b + 1; i < e; ++i) m = std::max(m, scores[i]);
-        if (m <= floor) continue;
-        for (uint64_t i = b; i < e; ++i) {
-            const int32_t s = scores[i];
-            if (s <= floor) continue;
-            if ((int)heap.size() < k) {
-                heap.emplace_back(s, (int64_t)i);
-                std::push_heap(heap.begin(), heap.end(), worse);
-            } else {
-                std::pop_heap(heap.begin(), heap.end(), worse);
-                heap.back() = {s, (int64_t)i};
-                std::push_heap(heap.begin(), heap.end(), worse);
+    // the best k of [lo, hi) as a min-heap of the current k best: once it is
+    // full a score must beat its weakest entry, and blocks of 64 whose
+    // maximum cannot are skipped
+    auto part = [&](uint64_t lo, uint64_t hi, std::vector<Entry>& heap) {
+        heap.clear();
+        heap.reserve(k + 1);
+        int32_t floor = 0;   // scores <= floor cannot enter
+        constexpr uint64_t kBlock = 64;
+        for (uint64_t b = lo; b < hi && k > 0; b += kBlock) {
+            const uint64_t e = std::min(hi, b + kBlock);
+            int32_t m = scores[b];
+            for (uint64_t i = b + 1; i < e; ++i) m = std::max(m, scores[i]);
+            if (m <= floor) continue;
+            for (uint64_t i = b; i < e; ++i) {
+                const int32_t s = scores[i];
+                if (s <= floor) continue;
+                if ((int)heap.size() < k) {
+                    heap.emplace_back(s, (int64_t)i);
+                    std::push_heap(heap.begin(), heap.end(), worse);
+                } else {
+                    std::pop_heap(heap.begin(), heap.end(), worse);
+                    heap.back() = {s, (int64_t)i};
+                    std::push_heap(heap.begin(), heap.end(), worse);
+                }
+                if ((int)heap.size() == k) floor = heap.front().first;
             }
-            if ((int)heap.size() == k) floor = heap.front().first;
         }
+    };
+    std::vector<Entry> best;
+    // large score vectors (a search over millions of sequences) in 16 parts
+    constexpr uint64_t kPartMin = uint64_t(1) << 18;
+    const uint64_t hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    const uint64_t parts = std::min<uint64_t>(hw, std::max<uint64_t>(1, n / kPartMin));
+    if (parts <= 1 || k == 0) {
+        part(0, n, best);
+    } else {
+        std::vector<std::vector<Entry>> heaps(parts);
+        std::vector<std::thread> team;
+        for (uint64_t p = 1; p < parts; ++p)
+            team.emplace_back([&, p] { part(n * p / parts, n * (p + 1) / parts, heaps[p]); });
+        part(0, n / parts, heaps[0]);
+        for (auto& t : team) t.join();
+        for (auto& h : heaps) best.insert(best.end(), h.begin(), h.end());
     }
-    std::sort(heap.begin(), heap.end(), worse);
+    std::sort(best.begin(), best.end(), worse);
     for (int t = 0; t < k; ++t) {
-        out_score[t] = t < (int)heap.size() ? heap[t].first : 0;
-        out_index[t] = t < (int)heap.size() ? heap[t].second : 0;
+        out_score[t] = t < (int)best.size() ? best[t].first : 0;
+        out_index[t] = t < (int)best.size() ? best[t].second : 0;
     }
     return SWH_OK;
 }

@@ -56,3 +56,15 @@ def test_host_topk_matches_resultlist():
         a = topk(s, k)
         b = O.topk(s, k)
         assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]), (n, k, hi)
+
+
+def test_host_topk_large_vectors_in_parts():
+    """Score vectors of a million and more are taken in parts on several
+    threads; the result is the same total order (score desc, index asc)."""
+    rng = np.random.Generator(np.random.PCG64(2))
+    for n, hi, k in ((1_500_000, 90, 20), (2_000_003, 4, 50), (600_000, 70000, 10),
+                     (524_289, 2, 100), (1_048_576, 1, 5)):
+        s = rng.integers(-5, hi, size=n).astype(np.int32)
+        a = topk(s, k)
+        b = O.topk(s, k)
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]), (n, hi, k)
