@@ -680,14 +680,17 @@ int enqueue_tasks(swh_ctx* c, swh_db* db, const Task* t, int nt, const Shape& sh
     // length.  Duration model in VALU ops per lane: a lane item runs its first
     // (longest) block's rows through every strip at 9.5 ops per packed cell
     // plus 13 per strip row; a group item runs its first pair's rows (plus the
-    // lane skew) through every pass at 9.5 ops per cell plus 17 per step.
+    // lane skew of group_step_rows rows per lane) through every pass at 9.5
+    // ops per cell plus 17 per row.
     const int n_items = items_g + items_l;
     auto item_cost = [&](int list, int k) -> double {
         if (list < nt) {
             const QueryDesc& Gd = hd[nt + list];
             const int p = Gd.blk_begin + k * kWavesPerWG * (kWave / Gd.group);
             const int npass = (Gd.qpad / sh.Wg + Gd.group - 1) / Gd.group;
-            return (double)npass * (db->blocks[p].rows + Gd.group - 1) * (9.5 * sh.Wg + 17.0);
+            const int nr = group_step_rows(sh.Wl, sh.Wg);
+            return (double)npass * (db->blocks[p].rows + nr * (Gd.group - 1)) *
+                   (9.5 * sh.Wg + 17.0);
         }
         const QueryDesc& L = hd[list - nt];
         const int b = L.blk_begin + k * kWavesPerWG;

@@ -10,6 +10,11 @@ constexpr int kWave = 64;            // wavefront width on CDNA4
 constexpr int kWavesPerWG = 4;       // 256-thread workgroups
 constexpr int kLaneSeqs = 2 * kWave; // sequences per lane-mode block (lo/hi halves)
 constexpr int kProfRows = 24;        // residue codes 0..23 (23 = pad)
+// Database rows per group-mode step of k_scan<WL, WG>: two where the
+// kernel's register budget holds a second row's profile (256 VGPRs at
+// WL >= 24; at WL = 16 the 168 VGPRs of three workgroups per CU hold it beside
+// 8-column groups only).  With two, the rows' dependent chains interleave.
+constexpr int group_step_rows(int wl, int wg) { return (wl == 16 && wg == 16) ? 1 : 2; }
 constexpr int kSegCols = 1344;       // query columns per LDS profile segment (multiple of 16, 24 and 32;
                                      // 24 rows x (1344+8) int16 = 64,896 B of LDS)
 constexpr int kProfPad = 8;          // extra int16 per LDS row: rows start 16 B apart mod 128 B

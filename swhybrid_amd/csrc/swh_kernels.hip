@@ -443,16 +443,20 @@ __device__ __forceinline__ void lane_item(const ScanArgs& A, const QueryDesc& Qd
 // ---------------------------------------------------------------------------
 // Group mode for long database sequences.  G lanes share one sequence pair:
 // lane g of a group keeps query columns [pass*G*W + g*W, +W) in registers and
-// at step t works on row t-g; the H and F of its last column reach lane g+1
-// one step later through a wave-wide DPP shift (wave_shr:1), so the pair is
-// swept as a pipeline along the database with one row of skew per lane.  The
+// at step t works on rows NR(t-g) .. NR(t-g)+NR-1 (NR = group_step_rows, one
+// or two); the H and F of its last column in each row reach lane g+1 one step
+// later through a wave-wide DPP shift (wave_shr:1), so the pair is swept as a
+// pipeline along the database with NR rows of skew per lane.  The
 // first lane of a group reads the previous pass's right edge from the border
 // array, the last active lane writes it for the next pass.
 __device__ __forceinline__ uint32_t wave_shr1(uint32_t v) {
     return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x138, 0xf, 0xf, false);
 }
 
-template <int W>
+template <int WL, int WG>
+constexpr int kGroupStepRows = group_step_rows(WL, WG);
+
+template <int W, int NR>
 __device__ __forceinline__ void group_item(const ScanArgs& A, const QueryDesc& Qd, int local,
                                            short* lds, uint2* slot, uint32_t (*s_red)[kWave]) {
     const int wave = threadIdx.x >> 6;
@@ -495,71 +499,102 @@ __device__ __forceinline__ void group_item(const ScanArgs& A, const QueryDesc& Q
             e[c] = Z;
         }
         s2 diag0 = Z;
-        uint32_t ch = kZero16x2, cf = kZero16x2;   // this lane's last (H, F)
-        const int steps = rmax > 0 ? (int)rmax + nl - 1 : 0;
-        // residue words and border entries are loaded 4 steps ahead; rows
+        // NR rows per step: at step t lane g works on rows NR(t-g) .. NR(t-g)+NR-1
+        // (block rows are a multiple of 4, so all or none are in range); with
+        // two, the second row's columns interleave with the first's in the
+        // schedule, which the F chain of a single row leaves idle
+        uint32_t ch[NR], cf[NR];   // this lane's last (H, F) of each row of its step
+#pragma unroll
+        for (int j = 0; j < NR; ++j) ch[j] = cf[j] = kZero16x2;
+        constexpr int U = 4 / NR;   // steps per 4 rows
+        const int steps = rmax > 0 ? (int)(rmax / NR) + nl - 1 : 0;
+        // residue words and border entries are loaded 4 rows ahead; rows
         // outside the pair read a clamped word (any code is a valid LDS row)
         const bool reads_border = has_in && gl == 0 && has_pair;
         const int rows = (int)bd.rows;
         const int last_row = max(rows - 1, 0);
-        uint2 wa[4], ba[4], wb[4], bb[4];
+        auto word_at = [&](int t) { return pw[min(max(NR * (t - gl), 0), last_row) >> 2]; };
+        uint2 wa[U], wb[U], ba[U][NR], bb[U][NR];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            wa[u] = pw[min(max(u - gl, 0), last_row) >> 2];
-            ba[u] = make_uint2(kZero16x2, kZero16x2);
-            if (reads_border && u < rows) ba[u] = bord[u];
+        for (int u = 0; u < U; ++u) {
+            wa[u] = word_at(u);
             wb[u] = wa[u];
-            bb[u] = ba[u];
+#pragma unroll
+            for (int j = 0; j < NR; ++j) {
+                ba[u][j] = make_uint2(kZero16x2, kZero16x2);
+                if (reads_border && NR * u + j < rows) ba[u][j] = bord[NR * u + j];
+                bb[u][j] = ba[u][j];
+            }
         }
-        // the profile rows of a step are loaded while the step before it computes
+        // a step's first profile row is loaded while the step before it
+        // computes, its other row at the start of the step
         RawProfile<W> rp;
         {
-            const int sh = 8 * ((-gl) & 3);
+            const int sh = 8 * ((-NR * gl) & 3);
             load_profile<W>(rp, lds, A.ps_group, (wa[0].x >> sh) & 0xff, (wa[0].y >> sh) & 0xff,
                             jj0);
         }
-        // 4 steps from (wc, bc) while the next 4 steps' words and borders load
+        // U steps from (wc, bc) while the next U steps' words and borders load
         // into (wn, bn); unrolled twice with the two sets swapping roles
-        auto quad = [&](int t0, const uint2 (&wc)[4], const uint2 (&bc)[4], uint2 (&wn)[4],
-                        uint2 (&bn)[4]) {
+        auto unit = [&](int t0, const uint2 (&wc)[U], const uint2 (&bc)[U][NR], uint2 (&wn)[U],
+                        uint2 (&bn)[U][NR]) {
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                wn[u] = pw[min(max(t0 + 4 + u - gl, 0), last_row) >> 2];
-                const int tn = t0 + 4 + u;
-                if (reads_border && tn < rows) bn[u] = bord[tn];
+            for (int u = 0; u < U; ++u) {
+                const int tn = t0 + U + u;
+                wn[u] = word_at(tn);
+#pragma unroll
+                for (int j = 0; j < NR; ++j)
+                    if (reads_border && NR * tn + j < rows) bn[u][j] = bord[NR * tn + j];
             }
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const int row = t0 + u - gl;
+            for (int u = 0; u < U; ++u) {
+                const int r0 = NR * (t0 + u - gl);
                 const RawProfile<W> cur = rp;
-                const uint2 nw = u < 3 ? wc[u + 1] : wn[0];
-                const int sh = 8 * ((row + 1) & 3);
+                std::conditional_t<NR == 2, RawProfile<W>, char> rq;   // the second row
+                if constexpr (NR == 2) {
+                    const int sh = 8 * ((r0 + 1) & 3);
+                    load_profile<W>(rq, lds, A.ps_group, (wc[u].x >> sh) & 0xff,
+                                    (wc[u].y >> sh) & 0xff, jj0);
+                }
+                const uint2 nw = u + 1 < U ? wc[u + 1] : wn[0];
+                const int sh = 8 * ((r0 + NR) & 3);
                 load_profile<W>(rp, lds, A.ps_group, (nw.x >> sh) & 0xff, (nw.y >> sh) & 0xff,
                                 jj0);
                 __builtin_amdgcn_sched_barrier(0);   // issue the loads before the step's work
-                // every lane takes part in the shift (no lane may be masked off here)
-                const uint32_t hx = wave_shr1(ch);
-                const uint32_t fx = wave_shr1(cf);
-                const uint32_t hl_u = gl == 0 ? bc[u].x : hx;
-                const uint32_t fl_u = gl == 0 ? bc[u].y : fx;
-                if (on && row >= 0 && row < rows) {
-                    s2 P[W];
-                    interleave_profile<W>(P, cur);
-                    s2 fout;
-                    row_cells<W, s2>(h, e, P, diag0, as_s2(hl_u), as_s2(fl_u), Q, R, fout);
-                    S = row_max_into<W, s2>(S, h);
-                    ch = bits(h[W - 1]);
-                    cf = bits(fout);
-                    diag0 = as_s2(hl_u);
-                    if (has_out && gl == nl - 1) bord[row] = make_uint2(ch, cf);
+                // every lane takes part in the shifts (no lane may be masked off here)
+                uint32_t hl[NR], fl[NR];
+#pragma unroll
+                for (int j = 0; j < NR; ++j) {
+                    const uint32_t hx = wave_shr1(ch[j]);
+                    const uint32_t fx = wave_shr1(cf[j]);
+                    hl[j] = gl == 0 ? bc[u][j].x : hx;
+                    fl[j] = gl == 0 ? bc[u][j].y : fx;
+                }
+                if (on && r0 >= 0 && r0 < rows) {
+                    auto row = [&](const RawProfile<W>& raw, int j) {
+                        s2 P[W];
+                        interleave_profile<W>(P, raw);
+                        s2 fout;
+                        row_cells<W, s2>(h, e, P, diag0, as_s2(hl[j]), as_s2(fl[j]), Q, R, fout);
+                        S = row_max_into<W, s2>(S, h);
+                        ch[j] = bits(h[W - 1]);
+                        cf[j] = bits(fout);
+                        diag0 = as_s2(hl[j]);
+                    };
+                    row(cur, 0);
+                    if constexpr (NR == 2) row(rq, 1);
+                    if (has_out && gl == nl - 1) {
+#pragma unroll
+                        for (int j = 0; j < NR; ++j) bord[r0 + j] = make_uint2(ch[j], cf[j]);
+                    }
                 }
                 __builtin_amdgcn_sched_barrier(0);
             }
         };
-        for (int t0 = 0; t0 < steps; t0 += 8) {
-            quad(t0, wa, ba, wb, bb);
-            if (t0 + 4 >= steps) break;   // a uniform branch: no tail copy of the code
-            quad(t0 + 4, wb, bb, wa, ba);
+        for (int t0 = 0; t0 < steps; t0 += 2 * U) {
+            unit(t0, wa, ba, wb, bb);
+            if (t0 + U >= steps) break;   // a uniform branch: no tail copy of the code
+            unit(t0 + U, wb, bb, wa, ba);
         }
     }
     s_red[wave][lane] = bits(S);
@@ -613,7 +648,7 @@ void k_scan(ScanArgs A) {
     while (next_scan_item(A, s_buf, local, q, is_group)) {
         if (is_group) {
             const QueryDesc Qd = A.group_q[q];
-            group_item<WG>(A, Qd, local, lds, slot, s_red);
+            group_item<WG, kGroupStepRows<WL, WG>>(A, Qd, local, lds, slot, s_red);
             loaded_q = -1;   // the group item restaged the profile
         } else {
             const QueryDesc Qd = A.lane_q[q];
