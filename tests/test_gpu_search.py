@@ -185,6 +185,21 @@ def test_group_mode_shapes(eng, lanes, width):
     check_db(eng, db, q, min_len=300, group_lanes=lanes, group_width=width)
 
 
+@pytest.mark.parametrize("strip,placement,width", [
+    (16, "pair", 8), (16, "pair", 16), (16, "single", 8), (16, "single", 16), (24, "pair", 8),
+    (24, "single", 16), (24, "pair", 24), (24, "single", 24), (32, "single", 8),
+    (32, "single", 16)])
+def test_group_steps_every_kernel(eng, strip, placement, width):
+    """Group items in every scan kernel: one database row per step where the
+    kernel's register budget allows only that (16-column lanes with 16-column
+    groups), two elsewhere; 3 lanes per group make several passes, so the
+    pass borders of both rows of a step are written and read back."""
+    db = long_db(700 + strip + width, n=30)
+    q = G.queries([500], 701)[0]
+    check_db(eng, db, q, min_len=300, strip_width=strip, profile_mode=placement,
+             group_width=width, group_lanes=3)
+
+
 def test_group_mode_odd_count_and_overflow(eng):
     rng = np.random.Generator(np.random.PCG64(61))
     q = G.queries([900], 62)[0]
