@@ -113,7 +113,7 @@ bool DataPool::fillChunk(char* dst, batchInfo* info, size_t& infoSize, size_t& a
     };
     const size_t used = at.back();
     const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-    const unsigned nt = used >= (size_t(32) << 20) ? hw : 1u;
+    const unsigned nt = used >= (size_t(32) << 20) ? (fillThreads_ ? fillThreads_ : hw) : 1u;
     if (nt == 1 || b1 - b0 < 2 * nt) {
         copy(b0, b1);
     } else {

@@ -47,6 +47,9 @@ class DataPool {
     // Residue bytes at the end of the pool that CPU workers leave to the
     // device workers, so that a slow CPU chunk does not finish last.
     void setCpuReserve(size_t bytes) { cpuReserve_ = bytes; }
+    // Threads that copy a large chunk (default: the host's cores, at most 16);
+    // fewer leave cores to a CPU worker.
+    void setFillThreads(unsigned n) { fillThreads_ = n; }
 
   private:
     struct Batch {
@@ -65,6 +68,7 @@ class DataPool {
     size_t handedOutCpu_ = 0, handedOutDev_ = 0;   // per kind of worker
     double t0_ = 0;                                // first fill, seconds
     size_t cpuReserve_ = 0;
+    unsigned fillThreads_ = 0;   // 0: the default
     size_t totalPadded_ = 0;
     size_t maxBatchBytes_ = 0;
     std::mutex mu_;
