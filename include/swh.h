@@ -168,8 +168,10 @@ typedef struct swh_tune_entry {
     int qlen_max;
     int strip_width;          /* best lane-scan tile depth (16, 24, 32) */
     int profile_mode;         /* best profile placement (SWH_PROFILE_SINGLE / _PAIR) */
-    int group_lanes;          /* best lanes per long sequence pair (1..64) */
-    int group_width;          /* best group-mode tile depth (8, 16, 24) */
+    int group_lanes;          /* best lanes per long sequence pair (1..64), or 0:
+                                 chosen per launch within its pass budget */
+    int group_width;          /* best group-mode tile depth (8, 16, 24), or 0: the
+                                 built-in choice per launch (then group_lanes 0) */
     float lane_cost[SWH_TUNE_LANE_CONFIGS];   /* measured launch ms per lane column
                                                  (query length rounded up to 8, a
                                                  narrower last strip weighted by

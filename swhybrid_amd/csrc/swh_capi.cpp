@@ -594,7 +594,7 @@ Shape choose_shape(const swh_ctx* c, const int* qlens, int nq, const swh_search_
             std::vector<double> cols(c->tune.size(), 0.0);
             for (int i = 0; i < nq; ++i) cols[tune_bucket(c, qlens[i])] += qlens[i];
             const size_t b = std::max_element(cols.begin(), cols.end()) - cols.begin();
-            sh.Wg = c->tune[b].group_width;
+            if (c->tune[b].group_width > 0) sh.Wg = c->tune[b].group_width;   // 0: built-in
         }
         if (sh.Wg == 24 && sh.Wl != 24) sh.Wg = 16;   // a bucket tuned with other lanes
     }

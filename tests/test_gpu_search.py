@@ -297,7 +297,8 @@ def test_autotune_installs_a_table_and_scores_stay_exact():
     assert [t["qlen_max"] for t in ents] == [64, 400, 1500]
     for t in ents:
         assert t["strip_width"] in (16, 24, 32) and t["profile_mode"] in (1, 2)
-        assert t["group_width"] in (8, 16, 24) and 0 <= t["group_lanes"] <= 64
+        assert t["group_width"] in (0, 8, 16, 24) and 0 <= t["group_lanes"] <= 64
+        assert t["group_width"] != 0 or t["group_lanes"] == 0
         assert t["group_width"] != 24 or t["strip_width"] == 24
         assert all(c > 0 for c in t["lane_cost"])
     assert e.get_tuning() == ents
@@ -345,12 +346,13 @@ def test_tuning_table_validation():
     for bad in (dict(strip_width=20), dict(profile_mode=0), dict(group_width=12),
                 dict(group_width=24),                      # 24 needs 24-column lanes
                 dict(strip_width=32, profile_mode=2), dict(lane_cost=[0, 0, 0, 0, 0]),
-                dict(group_lanes=65)):
+                dict(group_lanes=65), dict(group_width=0, group_lanes=3)):
         with pytest.raises(Exception):
             e.set_tuning([{**good, **bad}])
     with pytest.raises(Exception):
         e.set_tuning([good, {**good, "qlen_max": 50}])     # not ascending
     e.set_tuning([good, {**good, "qlen_max": 200, "strip_width": 24, "group_width": 24}])
+    e.set_tuning([good, {**good, "qlen_max": 200, "group_width": 0}])   # built-in group mode
     e.set_tuning([good])
     e.close()
 
