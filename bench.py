@@ -104,6 +104,8 @@ def run_batched(eng, db, d, plain_gcups, group_min_len):
     log(f"autotune: {tune['seconds']:.1f} s, group_min_len {tune['group_min_len']}, "
         f"{[(t['qlen_max'], t['strip_width'], t['profile_mode'], t['group_lanes'], t['group_width']) for t in tune['entries']]}")
     want_min = tune["group_min_len"]
+    # the default layout's threshold grows with the database (at least 2048,
+    # 2,672 here): a tuned 2048 keeps it, a longer or shorter one re-lays out
     cur_min = group_min_len if group_min_len > 0 else 2048
     if want_min != cur_min:
         d.close()
