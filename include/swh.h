@@ -108,6 +108,11 @@ int swh_pinned_free(void* p);
  * Param scoring fields it reads (src/Param.h:59-117). */
 int swh_ctx_create(int device, const int32_t* matrix, int gap_open, int gap_extend,
                    swh_ctx** out);
+/* While a server is attached to a context (swh_server_create), its loop
+ * thread owns the context's stream and launch buffers: swh_ctx_destroy,
+ * swh_ctx_set_scoring, swh_search, swh_search_batch,
+ * swh_autotune and swh_set_tuning on that context, and swh_db_destroy of the
+ * served database, return SWH_ERR_STATE until the server is destroyed. */
 int swh_ctx_destroy(swh_ctx* ctx);
 int swh_ctx_set_scoring(swh_ctx* ctx, const int32_t* matrix, int gap_open, int gap_extend);
 
@@ -249,7 +254,11 @@ int swh_server_submit(swh_server* srv, const uint8_t* query, int qlen, int k, in
 int swh_server_wait(swh_server* srv, uint64_t id, int timeout_ms, int32_t* top_score,
                     int64_t* top_index, int32_t* scores, swh_request_info* info);
 int swh_server_get_stats(swh_server* srv, swh_server_stats* out);
-/* Finishes the requests already admitted, fails the waiting ones, stops. */
+/* Finishes the requests already admitted, fails the waiting ones with
+ * SWH_ERR_STATE (a thread blocked in swh_server_wait on one of them returns
+ * that status before destroy frees the server), stops, and detaches the
+ * server from its context.  A request whose device buffers alone exceed
+ * memory_budget is refused at submit (SWH_ERR_CAPACITY). */
 int swh_server_destroy(swh_server* srv);
 
 #ifdef __cplusplus

@@ -15,7 +15,14 @@
 
 namespace swhh {
 thread_local std::string g_err;
+
+int refuse_if_served(const swh_ctx* c, const char* what) {
+    if (c->server.load() != nullptr)
+        return fail(SWH_ERR_STATE, "%s: a server is attached to this context (destroy it first)",
+                    what);
+    return SWH_OK;
 }
+}  // namespace swhh
 using namespace swhh;
 
 extern "C" {
@@ -89,6 +96,8 @@ int swh_ctx_create(int device, const int32_t* matrix, int gap_open, int gap_exte
 
 int swh_ctx_destroy(swh_ctx* c) {
     if (!c) return SWH_OK;
+    int rc = refuse_if_served(c, "swh_ctx_destroy");
+    if (rc) return rc;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->upload) (void)hipStreamSynchronize(c->upload);
@@ -118,6 +127,10 @@ int swh_ctx_set_scoring(swh_ctx* c, const int32_t* matrix, int gap_open, int gap
     if (!c) return fail(SWH_ERR_ARG, "ctx is NULL");
     int rc = check_scoring(matrix, gap_open, gap_extend);
     if (rc) return rc;
+    if ((rc = refuse_if_served(c, "swh_ctx_set_scoring"))) return rc;
+    // a captured graph holds the old gaps and overflow threshold in its
+    // kernel arguments: it must not be replayed with the new scoring
+    c->graph.release();
     std::memcpy(c->matrix, matrix, sizeof(c->matrix));
     c->gap_open = gap_open;
     c->gap_extend = gap_extend;
@@ -359,6 +372,9 @@ int swh_db_create(swh_ctx* c, const uint8_t* residues, const uint64_t* offsets,
 
 int swh_db_destroy(swh_db* db) {
     if (!db) return SWH_OK;
+    if (db->served.load())
+        return fail(SWH_ERR_STATE, "swh_db_destroy: a server searches this database "
+                                   "(destroy the server first)");
     if (db->alt) swh_db_destroy(db->alt);
     if (db->ctx) {   // the buffers go to the context's spares (see BufCache)
         (void)hipSetDevice(db->ctx->device);
@@ -439,12 +455,29 @@ static double group_lane_use(int qlen, int W, int max_passes, int min_lanes) {
 // workgroups of 4 waves per CU) can keep a border of its longest pair for
 // each of its 64 / G groups within the context's border budget, so that more
 // passes never shrink the grid.
-static int group_min_lanes(const swh_ctx* c, uint32_t rows_max) {
+static int group_min_lanes(const swh_ctx* c, uint32_t rows_max, uint64_t border_budget) {
     const uint64_t per_group = (uint64_t)std::max(c->num_cus, 1) * 3 * kWavesPerWG *
                                sizeof(uint2) * std::max<uint32_t>(rows_max, 1);
-    const uint64_t m = std::max<uint64_t>(1, std::min<uint64_t>(kWave, c->border_budget / per_group));
+    const uint64_t m = std::max<uint64_t>(1, std::min<uint64_t>(kWave, border_budget / per_group));
     return (int)(kWave / (m + 1)) + 1;
 }
+
+// n elements in a fixed inline array when n <= N, else on the heap: the
+// launch planning of a few queries allocates nothing.
+template <typename T, size_t N>
+struct SmallBuf {
+    T local[N];
+    std::vector<T> heap;
+    T* p;
+    explicit SmallBuf(size_t n) : p(local) {
+        if (n > N) {
+            heap.resize(n);
+            p = heap.data();
+        }
+    }
+    T& operator[](size_t i) { return p[i]; }
+    const T& operator[](size_t i) const { return p[i]; }
+};
 
 static int group_pass_budget(const swh_ctx* c, uint64_t launch_cells, uint32_t rows_max, int W) {
     if (rows_max == 0) return kWave;
@@ -582,7 +615,7 @@ Shape choose_shape(const swh_ctx* c, const int* qlens, int nq, const swh_search_
             for (int i = 0; i < nq; ++i) cells += (uint64_t)qlens[i] * db_residues;
             const int p16 = group_pass_budget(c, cells, db->max_rows_pair, 16);
             const int p24 = group_pass_budget(c, cells, db->max_rows_pair, 24);
-            const int gmin = group_min_lanes(c, db->max_rows_pair);
+            const int gmin = group_min_lanes(c, db->max_rows_pair, c->border_budget);
             double u16 = 0, u24 = 0;
             for (int i = 0; i < nq; ++i) {
                 u16 += qlens[i] * group_lane_use(qlens[i], 16, p16, gmin);
@@ -591,9 +624,10 @@ Shape choose_shape(const swh_ctx* c, const int* qlens, int nq, const swh_search_
             if (u24 > u16) sh.Wg = 24;
         }
         if (!c->tune.empty()) {
-            std::vector<double> cols(c->tune.size(), 0.0);
+            double cols[SWH_TUNE_MAX] = {};
+            const int nb = std::min((int)c->tune.size(), SWH_TUNE_MAX);
             for (int i = 0; i < nq; ++i) cols[tune_bucket(c, qlens[i])] += qlens[i];
-            const size_t b = std::max_element(cols.begin(), cols.end()) - cols.begin();
+            const int b = (int)(std::max_element(cols, cols + nb) - cols);
             if (c->tune[b].group_width > 0) sh.Wg = c->tune[b].group_width;   // 0: built-in
         }
         if (sh.Wg == 24 && sh.Wl != 24) sh.Wg = 16;   // a bucket tuned with other lanes
@@ -613,13 +647,13 @@ int enqueue_tasks(swh_ctx* c, swh_db* db, const Task* t, int nt, const Shape& sh
     const int thr = (o.overflow_threshold > 0) ? std::min(o.overflow_threshold, exact_thr)
                                                : exact_thr;
     const int gap_oe = c->gap_open + c->gap_extend;
-    std::vector<QueryDesc> hd(3 * (size_t)nt);
+    SmallBuf<QueryDesc, 24> hd(3 * (size_t)nt);
     uint64_t launch_cells = 0;
     for (int i = 0; i < nt; ++i)
         launch_cells += (uint64_t)t[i].qlen * (range_residues(db, t[i].pair_begin, t[i].pair_end) +
                                                range_residues(db, t[i].lane_begin, t[i].lane_end));
     const int max_passes = group_pass_budget(c, launch_cells, db->max_rows_pair, sh.Wg);
-    const int min_lanes = group_min_lanes(c, db->max_rows_pair);
+    const int min_lanes = group_min_lanes(c, db->max_rows_pair, lr.border_budget);
     int items_l = 0, items_g = 0, max_qpad_l = 0, max_pass_g = 0, max_gpw = 1, max_pass_r = 0;
     int max_gpw_r = 0;   // lane groups per wave of the multi-pass rescores (0: none)
     bool lane_border = false, group_border = false;
@@ -704,29 +738,33 @@ int enqueue_tasks(swh_ctx* c, swh_db* db, const Task* t, int nt, const Shape& sh
             return cost != h.cost ? cost < h.cost : list > h.list;
         }
     };
-    std::vector<Head> heap;
-    heap.reserve(2 * (size_t)nt);
+    SmallBuf<Head, 16> heap(2 * (size_t)nt);
+    size_t nh = 0;
     for (int l = 0; l < 2 * nt; ++l)
         if ((l < nt ? hd[nt + l].n_items : hd[l - nt].n_items) > 0)
-            heap.push_back(Head{item_cost(l, 0), l, 0});
-    std::make_heap(heap.begin(), heap.end());
+            heap[nh++] = Head{item_cost(l, 0), l, 0};
+    std::make_heap(heap.p, heap.p + nh);
     const size_t desc_bytes = 3 * (size_t)nt * sizeof(QueryDesc);
     const size_t total_bytes = desc_bytes + (size_t)n_items * sizeof(uint2);
+    if ((lr.h_desc->fixed && total_bytes > lr.h_desc->bytes) ||
+        (lr.desc->fixed && total_bytes > lr.desc->bytes))
+        return fail(SWH_ERR_CAPACITY, "%zu bytes of work descriptors, above the fixed %zu",
+                    total_bytes, std::min(lr.h_desc->bytes, lr.desc->bytes));
     HIP_TRY(lr.h_desc->reserve(total_bytes));
     HIP_TRY(lr.desc->reserve(total_bytes));
-    std::memcpy(lr.h_desc->p, hd.data(), desc_bytes);
+    std::memcpy(lr.h_desc->p, hd.p, desc_bytes);
     uint2* hitems = (uint2*)((char*)lr.h_desc->p + desc_bytes);
-    for (int n = 0; !heap.empty(); ++n) {
-        std::pop_heap(heap.begin(), heap.end());
-        Head& h = heap.back();
+    for (int n = 0; nh > 0; ++n) {
+        std::pop_heap(heap.p, heap.p + nh);
+        Head& h = heap[nh - 1];
         hitems[n] = h.list < nt ? make_uint2((uint32_t)h.list, (uint32_t)h.k)
                                 : make_uint2((uint32_t)(h.list - nt) | kLaneItem, (uint32_t)h.k);
         const int n_list = h.list < nt ? hd[nt + h.list].n_items : hd[h.list - nt].n_items;
         if (++h.k < n_list) {
             h.cost = item_cost(h.list, h.k);
-            std::push_heap(heap.begin(), heap.end());
+            std::push_heap(heap.p, heap.p + nh);
         } else {
-            heap.pop_back();
+            --nh;
         }
     }
     *lr.desc_used = total_bytes;
@@ -750,7 +788,7 @@ int enqueue_tasks(swh_ctx* c, swh_db* db, const Task* t, int nt, const Shape& sh
     const int items = n_items;
     int grid = c->num_cus * scan_max_wg_per_cu(sh.Wl, sh.Wg, sh.pair, lds);
     if (per_wg)
-        grid = (int)std::min<uint64_t>(grid, std::max<uint64_t>(1, c->border_budget / per_wg));
+        grid = (int)std::min<uint64_t>(grid, std::max<uint64_t>(1, lr.border_budget / per_wg));
     grid = std::max(1, std::min(grid, items));
     // rescore: a lane group per sequence, one pass's profile columns in LDS;
     // only queries swept in more than one pass keep a border per group, and
@@ -761,10 +799,15 @@ int enqueue_tasks(swh_ctx* c, swh_db* db, const Task* t, int nt, const Shape& sh
     const uint64_t rstride = (uint64_t)max_gpw_r * lmax;
     const uint64_t r_per_wg = rstride * sizeof(uint2) * kWavesPerWG;
     int rgrid = c->num_cus * rescore_max_wg_per_cu(lds_r);
+    const uint64_t rbudget = std::min<uint64_t>(size_t(1) << 30, lr.border_budget);
     if (r_per_wg)
-        rgrid = (int)std::max<uint64_t>(1, std::min<uint64_t>(rgrid, (size_t(1) << 30) / r_per_wg));
-    HIP_TRY(lr.border->reserve(
-        std::max<uint64_t>({16, per_wg * (uint64_t)grid, r_per_wg * (uint64_t)rgrid})));
+        rgrid = (int)std::max<uint64_t>(1, std::min<uint64_t>(rgrid, rbudget / r_per_wg));
+    const uint64_t border_need =
+        std::max<uint64_t>({16, per_wg * (uint64_t)grid, r_per_wg * (uint64_t)rgrid});
+    if (lr.border->fixed && border_need > lr.border->bytes)
+        return fail(SWH_ERR_CAPACITY, "strip borders need %llu bytes for one workgroup, above "
+                    "the fixed %zu", (unsigned long long)border_need, lr.border->bytes);
+    HIP_TRY(lr.border->reserve(border_need));
 
     ScanArgs a{};
     a.words = (const uint32_t*)db->words.p;
@@ -894,7 +937,8 @@ int run_queries(swh_ctx* c, swh_db* db_in, const uint8_t* const* qs, const int* 
     if (o.use_graph) {
         // the database by id: a new one may reuse a destroyed one's address
         key = {(long long)db->uid, nq, sh.Wl, sh.Wg, sh.pair, (long long)c->tune_gen,
-               o.group_lanes, o.overflow_threshold,
+               o.group_lanes, o.overflow_threshold, c->gap_open, c->gap_extend,
+               c->max_score,
                pinned_out, (long long)(uintptr_t)(direct_out ? scores_host : nullptr),
                (long long)(uintptr_t)c->prof.p, (long long)(uintptr_t)c->scores.p,
                (long long)(uintptr_t)c->ovf.p, (long long)(uintptr_t)c->counters.p,
@@ -974,6 +1018,8 @@ int swh_search(swh_ctx* c, swh_db* db, const uint8_t* query, int qlen,
     if (!c || !db || !query || (!scores && db->nseq > 0))
         return fail(SWH_ERR_ARG, "NULL argument");
     if (db->ctx != c) return fail(SWH_ERR_ARG, "database belongs to another context");
+    int rc = refuse_if_served(c, "swh_search");
+    if (rc) return rc;
     swh_search_opts o{};
     if (opts) o = *opts;
     const uint8_t* qs[1] = {query};
@@ -985,6 +1031,8 @@ int swh_search_batch(swh_ctx* c, swh_db* db, const uint8_t* const* queries, cons
     if (!c || !db || nq < 0 || (nq > 0 && (!queries || !qlens || (!scores && db->nseq > 0))))
         return fail(SWH_ERR_ARG, "NULL argument");
     if (db->ctx != c) return fail(SWH_ERR_ARG, "database belongs to another context");
+    int rc = refuse_if_served(c, "swh_search_batch");
+    if (rc) return rc;
     swh_search_opts o{};
     if (opts) o = *opts;
     return run_queries(c, db, queries, qlens, nq, o, scores, stats);

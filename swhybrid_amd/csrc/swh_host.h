@@ -64,12 +64,23 @@ inline void parallel_for(int64_t n, F&& f) {
     for (auto& t : ts) t.join();
 }
 
-// Device buffer that only grows.
+// Device buffer that only grows.  A fixed buffer is a view of memory the
+// buffer does not own (a part of a static device array): it never
+// allocates, a larger request fails, and release() only forgets it.
 struct DevBuf {
     void* p = nullptr;
     size_t bytes = 0;
+    bool fixed = false;
+    static DevBuf view(void* at, size_t n) {
+        DevBuf b;
+        b.p = at;
+        b.bytes = n;
+        b.fixed = true;
+        return b;
+    }
     hipError_t reserve(size_t want) {
         if (want <= bytes) return hipSuccess;
+        if (fixed) return hipErrorOutOfMemory;
         if (p) {
             hipError_t e = hipFree(p);
             if (e != hipSuccess) return e;
@@ -81,7 +92,7 @@ struct DevBuf {
         return e;
     }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p && !fixed) (void)hipFree(p);
         p = nullptr;
         bytes = 0;
     }
@@ -155,8 +166,10 @@ struct BufCache {
 struct PinnedBuf {
     void* p = nullptr;
     size_t bytes = 0;
+    bool fixed = false;   // allocated once at set-up: a larger request fails
     hipError_t reserve(size_t want) {
         if (want <= bytes) return hipSuccess;
+        if (fixed) return hipErrorOutOfMemory;
         if (p) {
             (void)hipHostFree(p);
             p = nullptr;
@@ -227,6 +240,10 @@ struct swh_ctx {
     DevBuf stage_raw, stage_soff;       // its staging buffers, reused
     std::vector<swh_tune_entry> tune;   // empty: built-in shape choice
     uint64_t tune_gen = 0;              // bumped whenever the table changes
+    // A server's loop thread launches on this context's stream with its
+    // descriptor, border and counter buffers; while one is attached every
+    // other call that would use them is refused (SWH_ERR_STATE).
+    std::atomic<swh_server*> server{nullptr};
 };
 
 struct swh_db {
@@ -244,6 +261,7 @@ struct swh_db {
     uint64_t residues_pair = 0;                  // real residues in pair blocks
     std::vector<BlockDesc> blocks;
     DevBuf words, d_blocks, perm, sorted_len, sorted_blk;
+    std::atomic<bool> served{false};   // a server searches it: not destroyable
 };
 
 
@@ -299,10 +317,14 @@ struct LaunchRes {
     PinnedBuf* h_desc;
     DevBuf* border;
     size_t* desc_used;   // bytes of h_desc the last enqueue uploaded
+    uint64_t border_budget;   // bytes the strip borders may take (the grid shrinks to fit)
 };
 inline LaunchRes ctx_res(swh_ctx* c) {
-    return LaunchRes{c->stream, &c->desc, &c->h_desc, &c->border, &c->desc_used};
+    return LaunchRes{c->stream, &c->desc, &c->h_desc, &c->border, &c->desc_used,
+                     c->border_budget};
 }
+// SWH_ERR_STATE (with a message naming what) while a server is attached to c.
+int refuse_if_served(const swh_ctx* c, const char* what);
 
 int enqueue_tasks(swh_ctx* c, swh_db* db, const Task* t, int nt, const Shape& sh,
                   const swh_search_opts& o, uint32_t* counters, const LaunchRes& lr,

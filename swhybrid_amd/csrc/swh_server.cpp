@@ -61,6 +61,7 @@ struct swh_server {
     uint64_t next_id = 1;
     bool stop = false;        // no new admissions; the loop ends once nothing is active
     bool loop_done = false;   // the loop thread has returned
+    int waiters = 0;          // threads inside swh_server_wait (destroy waits for them)
     std::string loop_error;
     swh_server_stats stats{};
     std::thread loop, completer;
@@ -90,16 +91,24 @@ std::vector<int> cut_blocks(const swh_db* db, int b0, int b1, int n) {
     return cut;
 }
 
+// What a round ended with: work done, a clean stop (nothing active, the
+// server is stopping), or a device failure (loop_error says which).
+enum class Round { kDone, kStop, kFail };
+
 // One round: admit, launch the next slice of every active request, wait.
-// Returns false when the loop should end.
-bool run_round(swh_server* s) {
+Round run_round(swh_server* s) {
     swh_ctx* c = s->ctx;
     swh_db* db = s->db;
     std::vector<std::shared_ptr<Request>> batch;
     std::vector<std::shared_ptr<Request>> admitted;
     {
         std::unique_lock<std::mutex> lk(s->mu);
-        s->cv_work.wait(lk, [&] { return s->stop || !s->waiting.empty() || !s->active.empty(); });
+        // (with every slot held by requests awaiting completion there is
+        // nothing to do until the completion thread frees one)
+        s->cv_work.wait(lk, [&] {
+            return s->stop || !s->active.empty() ||
+                   (!s->waiting.empty() && !s->free_slots.empty());
+        });
         while (!s->stop && !s->waiting.empty() && !s->free_slots.empty()) {
             auto& r = s->waiting.front();
             if (s->bytes_in_use + r->bytes > s->opts.memory_budget && !s->active.empty()) break;
@@ -111,7 +120,7 @@ bool run_round(swh_server* s) {
             admitted.push_back(r);
             s->waiting.pop_front();
         }
-        if (s->active.empty()) return !s->stop;
+        if (s->active.empty()) return s->stop ? Round::kStop : Round::kDone;
         batch = s->active;
         s->stats.peak_inflight = std::max<int>(s->stats.peak_inflight, (int)batch.size());
         s->stats.waiting = (int)s->waiting.size();
@@ -126,7 +135,7 @@ bool run_round(swh_server* s) {
         if (hipMemcpyAsync((int16_t*)s->prof.p + (size_t)r->slot * s->prof_slot_elems, hp,
                            n * sizeof(int16_t), hipMemcpyHostToDevice, c->stream) != hipSuccess) {
             s->loop_error = "profile upload failed";
-            return false;
+            return Round::kFail;
         }
     }
     std::vector<Task> tasks(batch.size());
@@ -164,7 +173,7 @@ bool run_round(swh_server* s) {
                            (uint32_t*)s->counters.p, ctx_res(c), &st);
     if (rc != SWH_OK) {
         s->loop_error = g_err;
-        return false;
+        return Round::kFail;
     }
     // requests on their last slice: copy the scores out
     std::vector<std::shared_ptr<Request>> finished;
@@ -176,12 +185,12 @@ bool run_round(swh_server* s) {
                                nseq * sizeof(int32_t), hipMemcpyDeviceToHost,
                                c->stream) != hipSuccess) {
                 s->loop_error = "score copy failed";
-                return false;
+                return Round::kFail;
             }
         }
     if (hipStreamSynchronize(c->stream) != hipSuccess) {
         s->loop_error = "round failed on the device";
-        return false;
+        return Round::kFail;
     }
     const double round_ms = now_ms() - t_round;
     std::lock_guard<std::mutex> lk(s->mu);
@@ -201,12 +210,13 @@ bool run_round(swh_server* s) {
         s->to_complete.push_back(r);
     }
     if (!finished.empty()) s->cv_complete.notify_one();
-    return true;
+    return Round::kDone;
 }
 
 void loop_main(swh_server* s) {
     for (;;) {
-        bool ok = run_round(s);
+        const Round r = run_round(s);
+        const bool ok = r != Round::kFail;
         std::lock_guard<std::mutex> lk(s->mu);
         if (!ok || (s->stop && s->active.empty())) {
             if (!ok)
@@ -284,6 +294,19 @@ int swh_server_create(swh_ctx* c, swh_db* db, const swh_server_opts* opts, swh_s
     if (rc) return rc;
     HIP_TRY(hipSetDevice(c->device));
     auto s = std::make_unique<swh_server>();
+    // one server per context: its loop thread owns the context's stream and
+    // launch buffers until swh_server_destroy
+    swh_server* none = nullptr;
+    if (!c->server.compare_exchange_strong(none, s.get()))
+        return fail(SWH_ERR_STATE, "a server is already attached to this context");
+    auto detach = [&] {
+        c->server.store(nullptr);
+        db->served.store(false);
+    };
+    if (db->served.exchange(true)) {
+        c->server.store(nullptr);
+        return fail(SWH_ERR_STATE, "another server searches this database");
+    }
     s->ctx = c;
     s->db = db;
     s->opts = o;
@@ -320,6 +343,7 @@ int swh_server_create(swh_ctx* c, swh_db* db, const swh_server_opts* opts, swh_s
         s->counters.release();
         s->h_prof.release();
         s->h_scores.release();
+        detach();
         return fail(SWH_ERR_NOMEM, "server buffers: %s", hipGetErrorString(e));
     }
     for (int i = slots - 1; i >= 0; --i) s->free_slots.push_back(i);
@@ -344,6 +368,9 @@ int swh_server_submit(swh_server* s, const uint8_t* query, int qlen, int k, int 
     r->k = k;
     r->want_scores = want_scores != 0;
     r->bytes = request_bytes(s, qlen);
+    if (r->bytes > s->opts.memory_budget)
+        return fail(SWH_ERR_CAPACITY, "request needs %zu device bytes, above the server's "
+                    "memory_budget %llu", r->bytes, (unsigned long long)s->opts.memory_budget);
     std::lock_guard<std::mutex> lk(s->mu);
     if (s->stop) return fail(SWH_ERR_STATE, "server is stopping");
     r->id = s->next_id++;
@@ -363,8 +390,13 @@ int swh_server_wait(swh_server* s, uint64_t id, int timeout_ms, int32_t* top_sco
     if (it == s->all.end()) return fail(SWH_ERR_ARG, "unknown request %llu", (unsigned long long)id);
     std::shared_ptr<Request> r = it->second;
     auto ready = [&] { return r->done; };
+    s->waiters += 1;
+    bool got = true;
     if (timeout_ms < 0) s->cv_done.wait(lk, ready);
-    else if (!s->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), ready))
+    else got = s->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), ready);
+    s->waiters -= 1;
+    if (s->waiters == 0) s->cv_done.notify_all();   // a destroy may wait for this
+    if (!got)
         return fail(SWH_ERR_STATE, "request %llu not done after %d ms", (unsigned long long)id,
                     timeout_ms);
     s->all.erase(id);
@@ -402,6 +434,14 @@ int swh_server_destroy(swh_server* s) {
         s->cv_complete.notify_all();
     }
     if (s->completer.joinable()) s->completer.join();
+    {
+        // every request is done now (admitted ones finished, waiting ones
+        // failed with SWH_ERR_STATE): let threads blocked in swh_server_wait
+        // return before the server goes away
+        std::unique_lock<std::mutex> lk(s->mu);
+        s->cv_done.notify_all();
+        s->cv_done.wait(lk, [&] { return s->waiters == 0; });
+    }
     (void)hipSetDevice(s->ctx->device);
     s->scores.release();
     s->ovf.release();
@@ -409,6 +449,8 @@ int swh_server_destroy(swh_server* s) {
     s->counters.release();
     s->h_prof.release();
     s->h_scores.release();
+    s->db->served.store(false);
+    s->ctx->server.store(nullptr);
     delete s;
     return SWH_OK;
 }

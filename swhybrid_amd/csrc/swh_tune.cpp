@@ -62,6 +62,8 @@ extern "C" {
 
 int swh_set_tuning(swh_ctx* c, const swh_tune_entry* e, int n) {
     if (!c) return fail(SWH_ERR_ARG, "ctx is NULL");
+    int rc = refuse_if_served(c, "swh_set_tuning");
+    if (rc) return rc;
     if (n < 0 || n > SWH_TUNE_MAX) return fail(SWH_ERR_ARG, "n must be in 0..%d", SWH_TUNE_MAX);
     if (n > 0 && !e) return fail(SWH_ERR_ARG, "entries is NULL");
     for (int i = 0; i < n; ++i) {
@@ -104,6 +106,7 @@ int swh_autotune(swh_ctx* c, const uint8_t* residues, const uint64_t* offsets,
                  const uint32_t* lengths, uint64_t nseq, int sample_every,
                  const int* bucket_qlens, int nb, int qpb, int reps, swh_tune_result* out) {
     if (!c || !out) return fail(SWH_ERR_ARG, "ctx/out is NULL");
+    if (int rc0 = refuse_if_served(c, "swh_autotune")) return rc0;
     if (nseq == 0 || !residues || !offsets || !lengths)
         return fail(SWH_ERR_ARG, "the database sample is empty");
     if (!bucket_qlens || nb < 1 || nb > SWH_TUNE_MAX)

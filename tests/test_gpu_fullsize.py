@@ -133,3 +133,107 @@ def test_overflow_stress_full_size():
         assert got[i][planted].min() > 0
         idx = np.unique(np.concatenate([base, rescored[i], planted[:20]]))
         check_sample(q, db, got[i], idx, A.matrix("blosum50"), 13, 2)
+
+
+def test_c1_full_size_every_score():
+    """The CPU-runnable workload in full: 100k sequences (29.9M residues), one
+    query of 256, BLOSUM62 10/2.  Every score and the top-10 (score desc,
+    index asc) equal the oracle."""
+    db = G.c1_db()
+    q = G.c1_query()
+    e = Engine("blosum62", 10, 2)
+    d = e.load_db(db)
+    got, st = e.search(d, q)
+    d.close()
+    e.close()
+    assert st["cells"] == db.total * len(q) and db.n == 100_000
+    want = O.search_i32(q, db, A.matrix("blosum62"), 10, 2)
+    bad = np.flatnonzero(got != want)
+    assert bad.size == 0, f"{bad.size} mismatches at {bad[:5]}"
+    s1, i1 = topk(got, 10)
+    s2, i2 = O.topk(want, 10)
+    assert np.array_equal(s1, s2) and np.array_equal(i1, i2)
+
+
+C3_TUNE_BUCKETS = (96, 192, 384, 768, 1536, 3072, 5000)
+
+
+def test_c3_batched_tuned_graph_replay(c2):
+    """64 queries of log-uniform lengths in [50, 5000] over the 5M-sequence
+    database in one launch, after the autotuner installed its table (and
+    re-laid out the database if it chose another group threshold): the eager
+    call, then its hipGraph replay, each row equal to the plain one-query
+    search of the same query on an untuned engine; a sample against the
+    oracle."""
+    db, e0, d0 = c2
+    qs = G.c3_queries()
+    e = Engine("blosum62", 10, 2)
+    tune = e.autotune(db, C3_TUNE_BUCKETS, sample_every=8, queries_per_bucket=4)
+    assert len(e.get_tuning()) == len(C3_TUNE_BUCKETS)
+    gmin = tune["group_min_len"]
+    d = e.load_db(db, group_min_len=-1 if gmin >= 2**31 else gmin)
+    out = e.pinned_scores(db.n, len(qs))
+    _, st0 = e.search_batch(d, qs, use_graph=1, out=out)     # eager, then captured
+    eager = out.copy()
+    out[:] = -9
+    _, st1 = e.search_batch(d, qs, use_graph=1, out=out)     # replayed
+    assert st1["cells"] == st0["cells"] == db.total * sum(len(q) for q in qs)
+    assert np.array_equal(out, eager), "graph replay differs from the eager call"
+    del out
+    d.close()
+    e.close()
+    row = np.empty(db.n, dtype=np.int32)
+    for i, q in enumerate(qs):
+        e0.search(d0, q, out=row)
+        assert np.array_equal(eager[i], row), f"query {i} (length {len(q)})"
+    rng = np.random.Generator(np.random.PCG64(7006))
+    lens = np.array([len(q) for q in qs])
+    full = set(np.argsort(lens)[[0, -1]].tolist()) | set(rng.choice(len(qs), 6, replace=False).tolist())
+    for i in range(len(qs)):
+        idx = (sample_idx(db, rng, n_random=1500, n_longest=20) if i in full
+               else np.sort(rng.choice(db.n, size=200, replace=False)))
+        check_sample(qs[i], db, eager[i], idx, A.matrix("blosum62"), 10, 2)
+
+
+def test_c4_server_poisson_on_resident_5m(c2):
+    """The serving loop on the resident 5M-sequence database, requests of
+    log-normal length (mean 400) arriving as a Poisson process at 5 QPS: each
+    request's full score vector and top-20 equal its plain search alone, and a
+    sample of each equals the oracle."""
+    import threading
+    import time
+
+    from swhybrid_amd.engine import Server
+    db, e0, d0 = c2
+    arrivals, qs = G.c4_requests(16, 5.0)
+    e = Engine("blosum62", 10, 2)
+    d = e.load_db(db)
+    srv = Server(e, d)
+    ids = [None] * len(qs)
+    t0 = time.perf_counter()
+
+    def feed():
+        for i, (a, q) in enumerate(zip(arrivals, qs)):
+            delay = t0 + a - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            ids[i] = srv.submit(q, k=20, want_scores=True)
+
+    th = threading.Thread(target=feed)
+    th.start()
+    th.join()
+    res = [srv.wait(i) for i in ids]
+    st = srv.stats()
+    srv.close()
+    d.close()
+    e.close()
+    assert st["completed"] == len(qs)
+    rng = np.random.Generator(np.random.PCG64(7007))
+    row = np.empty(db.n, dtype=np.int32)
+    for i, (q, r) in enumerate(zip(qs, res)):
+        e0.search(d0, q, out=row)
+        assert np.array_equal(r["scores"], row), f"request {i} differs from its search alone"
+        s, ix = topk(row, 20)
+        assert np.array_equal(r["top_score"], s) and np.array_equal(r["top_index"], ix), i
+        check_sample(q, db, r["scores"], np.sort(rng.choice(db.n, 300, replace=False)),
+                     A.matrix("blosum62"), 10, 2)

@@ -282,6 +282,31 @@ def test_graph_replays_match_eager(eng):
     d.close()
 
 
+def test_graph_not_replayed_after_set_scoring():
+    """A captured graph holds the gaps and the overflow threshold in its kernel
+    arguments: after set_scoring the same call must not replay it.  Capture
+    at 10/2, switch to 11/1 (and then to BLOSUM50 13/2), call again with the
+    same lengths: the scores follow the new scoring."""
+    e = Engine("blosum62", 10, 2)
+    rng = np.random.Generator(np.random.PCG64(93))
+    db = long_db(94, n=150, lo=20, hi=2500)
+    d = e.load_db(db, group_min_len=1200)
+    qs = [G.random_residues(rng, m) for m in (60, 700)]
+    for rep in range(2):                           # eager + capture, then a replay
+        got, _ = e.search_batch(d, qs, use_graph=1)
+        for i, q in enumerate(qs):
+            assert np.array_equal(got[i], O.search_i32(q, db, e.matrix, 10, 2)), (rep, i)
+    for matrix, go, ge in (("blosum62", 11, 1), ("blosum50", 13, 2), ("blosum62", 10, 2)):
+        e.set_scoring(matrix, go, ge)
+        for rep in range(2):
+            got, _ = e.search_batch(d, qs, use_graph=1)
+            for i, q in enumerate(qs):
+                want = O.search_i32(q, db, A.matrix(matrix), go, ge)
+                assert np.array_equal(got[i], want), (matrix, go, ge, rep, i)
+    d.close()
+    e.close()
+
+
 # --- autotuner ---------------------------------------------------------------------
 
 def test_autotune_installs_a_table_and_scores_stay_exact():

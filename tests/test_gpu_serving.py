@@ -100,3 +100,73 @@ def test_rejects_and_timeouts(setup):
     r = srv.wait(rid, timeout_ms=60000)
     assert r["info"]["status"] == 0
     srv.close()
+
+
+def test_attached_server_refuses_other_calls(setup):
+    """While a server's loop thread owns the context's stream and launch
+    buffers, direct searches, rescoring, tuning and destroying the served
+    database are refused; after the server is destroyed they work again."""
+    _, db, _ = setup
+    eng = Engine("blosum62", 10, 2)
+    d = eng.load_db(db)
+    q = queries(1, 95)[0]
+    srv = Server(eng, d)
+    for call in (lambda: eng.search(d, q), lambda: eng.search_batch(d, [q, q]),
+                 lambda: eng.set_scoring("blosum62", 11, 1), lambda: eng.set_tuning([]),
+                 lambda: d.close()):
+        with pytest.raises(Exception, match="server"):
+            call()
+    with pytest.raises(Exception, match="already attached"):
+        Server(eng, d)
+    r = srv.wait(srv.submit(q, k=5))
+    srv.close()
+    got, _ = eng.search(d, q)
+    s, ix = topk(got, 5)
+    assert np.array_equal(r["top_score"], s) and np.array_equal(r["top_index"], ix)
+    d.close()
+    eng.close()
+
+
+def test_destroy_fails_waiting_requests_with_state(setup):
+    """A budget that holds one request at a time: destroy finishes the one on
+    the device and fails the waiting ones with SWH_ERR_STATE, also for a
+    thread already blocked waiting on one of them."""
+    from swhybrid_amd import _native as N
+    import ctypes as C
+    # a database and queries large enough that the first request is still
+    # on the device (about 0.3 s) while the waiters block and destroy runs
+    rng = np.random.Generator(np.random.PCG64(98))
+    db = G.SeqDB.from_lengths(rng, G.lognormal_lengths(rng, 1_000_000, 350, 0.7, 20, 2000))
+    eng = Engine("blosum62", 10, 2)
+    d = eng.load_db(db)
+    qs = G.queries([6000] * 4, 96)
+    srv = Server(eng, d, memory_budget=db.n * 8 + (1 << 20), slices=4)
+    ids = [srv.submit(q, k=5) for q in qs]
+    codes = {}
+
+    def waiter(i):
+        ts = np.zeros(5, dtype=np.int32)
+        ti = np.zeros(5, dtype=np.int64)
+        info = N.SwhRequestInfo()
+        codes[i] = N.lib().swh_server_wait(srv.handle, C.c_uint64(ids[i]), -1, N.ptr(ts),
+                                           N.ptr(ti), None, C.byref(info))
+
+    ts = [threading.Thread(target=waiter, args=(i,)) for i in range(len(ids))]
+    for t in ts:
+        t.start()
+    time.sleep(0.05)
+    srv.close()
+    for t in ts:
+        t.join()
+    d.close()
+    eng.close()
+    assert codes[0] == 0
+    assert set(codes.values()) <= {0, -4} and -4 in codes.values(), codes
+
+
+def test_request_above_budget_refused_at_submit(setup):
+    eng, db, d = setup
+    srv = Server(eng, d, memory_budget=db.n * 4 // 2)
+    with pytest.raises(Exception, match="memory_budget"):
+        srv.submit(queries(1, 97)[0], k=5)
+    srv.close()
