@@ -237,6 +237,13 @@ class Server:
             N.check(N.lib().swh_server_destroy(self.handle))
             self.handle = None
 
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+        return False
+
     def __del__(self):
         try:
             self.close()

@@ -31,12 +31,13 @@ def queries(n, seed):
     return [G.random_residues(rng, int(m)) for m in G.lognormal_lengths(rng, n, 300, 0.7, 30, 1500)]
 
 
-def test_alone_equals_in_the_mix(setup):
+def test_alone_equals_in_the_mix(setup, request):
     eng, db, d = setup
     qs = queries(12, 92)
     target = qs[0]
     want = O.search_i32(target, db, eng.matrix, 10, 2)
     srv = Server(eng, d, slices=3)
+    request.addfinalizer(srv.close)
     alone = srv.wait(srv.submit(target, k=20, want_scores=True))
     assert np.array_equal(alone["scores"], want)
     # now the same query among others, submitted from several threads
@@ -61,20 +62,30 @@ def test_alone_equals_in_the_mix(setup):
     srv.close()
 
 
+def request_bytes(eng, d, q, **opts):
+    """Device bytes one request needs, as the server's refusal states them."""
+    import re
+    with Server(eng, d, memory_budget=1, **opts) as srv:
+        with pytest.raises(Exception, match="memory_budget") as exc:
+            srv.submit(q, k=5)
+    return int(re.search(r"needs (\d+) device bytes", str(exc.value)).group(1))
+
+
 def test_memory_admission_serialises(setup):
     eng, db, d = setup
-    qs = queries(5, 93)
+    rng = np.random.Generator(np.random.PCG64(93))
+    qs = [G.random_residues(rng, 400) for _ in range(5)]   # equal lengths: equal bytes
     # a budget that holds one request at a time: later ones wait for memory
-    srv = Server(eng, d, memory_budget=db.n * 4 + 1, slices=2)
-    ids = [srv.submit(q, k=5) for q in qs]
-    res = [srv.wait(i) for i in ids]
-    st = srv.stats()
+    need = request_bytes(eng, d, qs[0], slices=2)
+    with Server(eng, d, memory_budget=need + 1, slices=2) as srv:
+        ids = [srv.submit(q, k=5) for q in qs]
+        res = [srv.wait(i) for i in ids]
+        st = srv.stats()
     assert st["peak_inflight"] == 1
     for q, r in zip(qs, res):
         s, ix = O.topk(O.search_i32(q, db, eng.matrix, 10, 2), 5)
         assert np.array_equal(r["top_score"], s) and np.array_equal(r["top_index"], ix)
         assert r["info"]["admit_ms"] >= r["info"]["submit_ms"]
-    srv.close()
 
 
 def test_slices_and_forced_rescore(setup):
@@ -82,24 +93,22 @@ def test_slices_and_forced_rescore(setup):
     q = queries(1, 94)[0]
     want = O.search_i32(q, db, eng.matrix, 10, 2)
     for slices in (1, 5):
-        srv = Server(eng, d, slices=slices, overflow_threshold=40)
-        r = srv.wait(srv.submit(q, k=10, want_scores=True))
+        with Server(eng, d, slices=slices, overflow_threshold=40) as srv:
+            r = srv.wait(srv.submit(q, k=10, want_scores=True))
         assert np.array_equal(r["scores"], want)
         assert r["info"]["rounds"] == slices
-        srv.close()
 
 
 def test_rejects_and_timeouts(setup):
     eng, db, d = setup
-    srv = Server(eng, d, max_qlen=100, max_k=5)
-    with pytest.raises(Exception, match="max_qlen"):
-        srv.submit(np.zeros(101, dtype=np.uint8))
-    with pytest.raises(Exception):
-        srv.submit(np.zeros(10, dtype=np.uint8), k=6)
-    rid = srv.submit(np.arange(20, dtype=np.uint8) % 20, k=5)
-    r = srv.wait(rid, timeout_ms=60000)
+    with Server(eng, d, max_qlen=100, max_k=5) as srv:
+        with pytest.raises(Exception, match="max_qlen"):
+            srv.submit(np.zeros(101, dtype=np.uint8))
+        with pytest.raises(Exception):
+            srv.submit(np.zeros(10, dtype=np.uint8), k=6)
+        rid = srv.submit(np.arange(20, dtype=np.uint8) % 20, k=5)
+        r = srv.wait(rid, timeout_ms=60000)
     assert r["info"]["status"] == 0
-    srv.close()
 
 
 def test_attached_server_refuses_other_calls(setup):
@@ -140,7 +149,8 @@ def test_destroy_fails_waiting_requests_with_state(setup):
     eng = Engine("blosum62", 10, 2)
     d = eng.load_db(db)
     qs = G.queries([6000] * 4, 96)
-    srv = Server(eng, d, memory_budget=db.n * 8 + (1 << 20), slices=4)
+    need = request_bytes(eng, d, qs[0], slices=4)
+    srv = Server(eng, d, memory_budget=need + need // 2, slices=4)
     ids = [srv.submit(q, k=5) for q in qs]
     codes = {}
 
@@ -166,7 +176,6 @@ def test_destroy_fails_waiting_requests_with_state(setup):
 
 def test_request_above_budget_refused_at_submit(setup):
     eng, db, d = setup
-    srv = Server(eng, d, memory_budget=db.n * 4 // 2)
-    with pytest.raises(Exception, match="memory_budget"):
-        srv.submit(queries(1, 97)[0], k=5)
-    srv.close()
+    with Server(eng, d, memory_budget=db.n * 4 // 2) as srv:
+        with pytest.raises(Exception, match="memory_budget"):
+            srv.submit(queries(1, 97)[0], k=5)
