@@ -110,7 +110,7 @@ int swh_ctx_create(int device, const int32_t* matrix, int gap_open, int gap_exte
                    swh_ctx** out);
 /* While a server is attached to a context (swh_server_create), its loop
  * thread owns the context's stream and launch buffers: swh_ctx_destroy,
- * swh_ctx_set_scoring, swh_search, swh_search_batch,
+ * swh_ctx_set_scoring, swh_search, swh_search_batch, swh_search_noalloc,
  * swh_autotune and swh_set_tuning on that context, and swh_db_destroy of the
  * served database, return SWH_ERR_STATE until the server is destroyed. */
 int swh_ctx_destroy(swh_ctx* ctx);
@@ -141,6 +141,25 @@ int swh_db_info(const swh_db* db, uint64_t* nseq, uint64_t* residues, uint64_t* 
  * (src/CUDAWorker.cu:224-246). */
 int swh_search(swh_ctx* ctx, swh_db* db, const uint8_t* query, int qlen,
                const swh_search_opts* opts, int32_t* scores, swh_stats* stats);
+
+/* swh_search without allocation: the same scores for the same arguments,
+ * with every device buffer of the call taken from a static scratch array of
+ * 256 MiB per device (a __device__ array of libswh.so, which each GPU the
+ * library runs on holds until the process ends) and its descriptor tables
+ * staged through 1 MiB of page-locked memory that swh_ctx_create allocated.
+ * The call itself makes no hipMalloc, hipFree or other allocating call.
+ * Limits: query length <= 8192; the scratch must hold the database's score
+ * vector and overflow list (8 bytes per sequence) plus the strip borders of
+ * at least one workgroup; the work descriptors must fit 1 MiB.  Beyond them
+ * the call fails with SWH_ERR_CAPACITY.  use_graph must be 0.  Calls on one
+ * device share its array and are serialised by a process-wide lock per
+ * device; the array keeps the last call's contents (every buffer the kernels
+ * read before writing is cleared or written by the call itself).
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under graph capture, tight serving loops, harnesses
+ * that forbid allocation). */
+int swh_search_noalloc(swh_ctx* ctx, swh_db* db, const uint8_t* query, int qlen,
+                       const swh_search_opts* opts, int32_t* scores, swh_stats* stats);
 
 /* Many queries in one persistent launch with a device work queue; scores is
  * nq rows of nseq int32 in query order.  No reference counterpart: the

@@ -23,7 +23,7 @@ EXPORTS = (
     "swh_db_info", "swh_search", "swh_search_batch", "swh_topk",
     "swh_server_create", "swh_server_submit", "swh_server_wait", "swh_server_get_stats",
     "swh_server_destroy", "swh_autotune", "swh_set_tuning", "swh_get_tuning",
-    "swh_pinned_alloc", "swh_pinned_free",
+    "swh_pinned_alloc", "swh_pinned_free", "swh_search_noalloc",
 )
 
 SWH_PROFILE_AUTO, SWH_PROFILE_SINGLE, SWH_PROFILE_PAIR = 0, 1, 2
@@ -147,6 +147,8 @@ def lib():
                               C.POINTER(C.c_uint64)]
     L.swh_search.argtypes = [vp, vp, vp, C.c_int, C.POINTER(SwhSearchOpts), vp,
                              C.POINTER(SwhStats)]
+    L.swh_search_noalloc.argtypes = [vp, vp, vp, C.c_int, C.POINTER(SwhSearchOpts), vp,
+                                     C.POINTER(SwhStats)]
     L.swh_search_batch.argtypes = [vp, vp, vp, vp, C.c_int, C.POINTER(SwhSearchOpts), vp,
                                    C.POINTER(SwhStats)]
     L.swh_topk.argtypes = [vp, C.c_uint64, C.c_int, vp, vp]

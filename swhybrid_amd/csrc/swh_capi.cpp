@@ -85,6 +85,14 @@ int swh_ctx_create(int device, const int32_t* matrix, int gap_open, int gap_exte
         swh_ctx_destroy(c);
         return fail(SWH_ERR_NOMEM, "counter allocation failed");
     }
+    // the allocation-free search: this device's scratch array (looked up once
+    // per device) and its own fixed page-locked descriptor staging
+    hipError_t se = scratch_address(device, &c->scratch, &c->scratch_bytes);
+    if (se != hipSuccess || c->na_h_desc.reserve(kNoAllocDescBytes) != hipSuccess) {
+        swh_ctx_destroy(c);
+        return fail(SWH_ERR_HIP, "scratch set-up failed: %s", hipGetErrorString(se));
+    }
+    c->na_h_desc.fixed = true;
     rc = swh_ctx_set_scoring(c, matrix, gap_open, gap_extend);
     if (rc) {
         swh_ctx_destroy(c);
@@ -115,6 +123,7 @@ int swh_ctx_destroy(swh_ctx* c) {
     c->desc.release();
     c->h_desc.release();
     c->h_ctr.release();
+    c->na_h_desc.release();
     for (auto& e : c->ev)
         if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);

@@ -65,7 +65,7 @@ inline void parallel_for(int64_t n, F&& f) {
 }
 
 // Device buffer that only grows.  A fixed buffer is a view of memory the
-// buffer does not own (a part of a static device array): it never
+// buffer does not own (a part of the static scratch array): it never
 // allocates, a larger request fails, and release() only forgets it.
 struct DevBuf {
     void* p = nullptr;
@@ -244,6 +244,12 @@ struct swh_ctx {
     // descriptor, border and counter buffers; while one is attached every
     // other call that would use them is refused (SWH_ERR_STATE).
     std::atomic<swh_server*> server{nullptr};
+    // The allocation-free search (swh_search_noalloc): this device's static
+    // scratch array and a page-locked descriptor staging area of fixed size,
+    // both set up by swh_ctx_create.
+    void* scratch = nullptr;
+    size_t scratch_bytes = 0;
+    PinnedBuf na_h_desc;
 };
 
 struct swh_db {

@@ -98,8 +98,22 @@ struct RescoreArgs {
     int gap_oe, gap_e;
 };
 
+// Device scratch of the allocation-free search entry point: one static
+// array per device (kScratchBytes), whose address is looked up once per
+// device with that device current.  The descriptor tables of such a search
+// are staged through kNoAllocDescBytes of page-locked memory that the
+// context allocates when it is created.
+constexpr size_t kScratchBytes = size_t(256) << 20;
+constexpr size_t kNoAllocDescBytes = size_t(1) << 20;
+constexpr int kNoAllocMaxQlen = 8192;
+hipError_t scratch_address(int device, void** out, size_t* bytes);
+
 // Launchers (swh_kernels.hip).
 hipError_t launch_reset(uint32_t* p, int n, hipStream_t s);
+// [24][pstride] int16 profile s(q_j, r) (zero beyond qlen) of a query held
+// in device memory, from a 32x32 int32 matrix in device memory.
+hipError_t launch_build_profile(const int32_t* matrix, const uint8_t* query, int qlen,
+                                int pstride, int16_t* out, hipStream_t s);
 // Interleaves raw residues (sequence id's codes at raw + soff[id]) into the
 // block layout; one workgroup per block.
 hipError_t launch_pack(const BlockDesc* blocks, int n_blocks, int n_pairs, const uint32_t* perm,

@@ -30,6 +30,7 @@
 // the device.
 #include "swh_internal.h"
 
+#include <mutex>
 #include <type_traits>
 
 namespace swh {
@@ -869,9 +870,57 @@ __global__ void k_reset(uint32_t* p, int n) {
     if (i < n) p[i] = 0;
 }
 
+// Query profile on the device: out[r * pstride + j] = matrix[q_j * 32 + r] for
+// j < qlen, 0 beyond (the host's build_profile, for callers that stage the
+// query in device memory).
+__global__ void k_build_profile(const int32_t* __restrict__ matrix,
+                                const uint8_t* __restrict__ query, int qlen, int pstride,
+                                int16_t* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= kProfRows * pstride) return;
+    const int r = i / pstride;
+    const int j = i - r * pstride;
+    out[i] = j < qlen ? (int16_t)matrix[query[j] * 32 + r] : (int16_t)0;
+}
+
+// The scratch array of the allocation-free search (swh_search_noalloc).  In
+// host code its name is only a stand-in for the device symbol: the memory is
+// reached through the address hipGetSymbolAddress returns for each device.
+static __device__ __align__(256) uint8_t scratch[kScratchBytes];
+
+hipError_t scratch_address(int device, void** out, size_t* bytes) {
+    constexpr int kMaxDevices = 64;
+    static std::mutex mu;
+    static void* addr[kMaxDevices] = {};
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> g(mu);
+    if (!addr[device]) {
+        int cur = -1;
+        hipError_t e = hipGetDevice(&cur);
+        if (e != hipSuccess) return e;
+        if (cur != device) return hipErrorInvalidDevice;   // the caller makes it current
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        if (!p) return hipErrorInvalidSymbol;
+        addr[device] = p;
+    }
+    *out = addr[device];
+    *bytes = kScratchBytes;
+    return hipSuccess;
+}
+
 // ---------------------------------------------------------------------------
 hipError_t launch_reset(uint32_t* p, int n, hipStream_t s) {
     hipLaunchKernelGGL(k_reset, dim3((n + 255) / 256), dim3(256), 0, s, p, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_build_profile(const int32_t* matrix, const uint8_t* query, int qlen,
+                                int pstride, int16_t* out, hipStream_t s) {
+    const int n = kProfRows * pstride;
+    hipLaunchKernelGGL(k_build_profile, dim3((n + 255) / 256), dim3(256), 0, s, matrix, query,
+                       qlen, pstride, out);
     return hipGetLastError();
 }
 
