@@ -93,7 +93,7 @@ void CPUWorker::search() {
         pieces.push_back(Piece{i, n});
         i += n;
     }
-    const uint8_t* q = reinterpret_cast<const uint8_t*>(query.data());
+    const uint8_t* q = reinterpret_cast<const uint8_t*>(query);
     const size_t base = compEntry->getBaseIdx();
     std::atomic<size_t> next{0};
     std::atomic<uint64_t> pushed{0};

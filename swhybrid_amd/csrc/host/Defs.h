@@ -13,4 +13,9 @@ struct batchInfo {
 };
 
 const int alphabetSize = 32;
+// The largest chunk a worker's DataPool buffer holds.  The reference's Worker
+// allocates exactly this many bytes up front (1 << 28, src/Defs.h:14,
+// src/Worker.cpp:59); this pipeline's Worker allocates bufferSize bytes at the
+// first fill, so its bound is only a sanity limit.
+const size_t maxReaderBufferSize = size_t(16) << 30;
 const int kPadCode = 23;

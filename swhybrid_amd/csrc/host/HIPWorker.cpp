@@ -13,46 +13,40 @@ const char* swh_fail(const char* what) {
 
 HIPWorker::HIPWorker(DataPool* loader, Param* params, RecalcWorker* recalcWorker, int deviceId,
                      size_t chunkBytes)
-    : Worker(loader, params, recalcWorker, 4, 128, sizeof(int)), deviceId_(deviceId) {
-    bufferSize = chunkBytes;
+    : Worker(loader, params, recalcWorker, 4, 128, sizeof(int)),
+      deviceId_(deviceId),
+      ctx_(NULL),
+      kernelMs_(0),
+      cells_(0) {
+    dbs_[0] = dbs_[1] = NULL;
+    bufferSize = chunkBytes < (size_t)maxReaderBufferSize ? chunkBytes
+                                                          : (size_t)maxReaderBufferSize;
 }
 
-HIPWorker::~HIPWorker() {
-    free();
-    freeChunk(dbSeqBuf);
-    dbSeqBuf = nullptr;
-}
-
-char* HIPWorker::allocChunk(size_t bytes) {
-    void* p = nullptr;
-    if (swh_pinned_alloc(bytes, &p) != SWH_OK) throw swh_fail("swh_pinned_alloc");
-    return static_cast<char*>(p);
-}
-
-void HIPWorker::freeChunk(char* p) {
-    if (p) (void)swh_pinned_free(p);
-}
+HIPWorker::~HIPWorker() { free(); }
 
 void HIPWorker::setQuery(const char* q, size_t qlen) {
-    Worker::setQuery(q, qlen);
     for (size_t j = 0; j < qlen; ++j)
         if ((unsigned char)q[j] > 23) throw "HIPWorker: query code above 23\n";
+    queryCodes_.assign(q, q + qlen);
+    query = queryCodes_.data();
+    queryLen = (int)qlen;
 }
 
 void HIPWorker::alloc() {
     if (swh_ctx_create(deviceId_, params->plain_matrix, params->gapOpen, params->gapExtend,
                        &ctx_) != SWH_OK)
         throw swh_fail("swh_ctx_create");
-    for (int i = 0; i < 2; ++i) entry[i].setEntry(nullptr, nullptr, nullptr);
+    for (int i = 0; i < 2; ++i) entry[i].setEntry(NULL, NULL, NULL);
 }
 
 void HIPWorker::free() {
-    for (auto& d : dbs_) {
-        if (d) swh_db_destroy(d);
-        d = nullptr;
+    for (int i = 0; i < 2; ++i) {
+        if (dbs_[i]) swh_db_destroy(dbs_[i]);
+        dbs_[i] = NULL;
     }
     if (ctx_) swh_ctx_destroy(ctx_);
-    ctx_ = nullptr;
+    ctx_ = NULL;
 }
 
 // Chunk -> resident packed database in the aux entry's slot.
@@ -74,38 +68,34 @@ void HIPWorker::packDB() {
             off += info[b].seqLen;
         }
     if (dbs_[slot]) swh_db_destroy(dbs_[slot]);
-    dbs_[slot] = nullptr;
-    swh_db_opts dopt{};
+    dbs_[slot] = NULL;
+    swh_db_opts dopt = {};
     dopt.queries_per_launch = 1;   // one query per search: one layout per chunk
     if (swh_db_create(ctx_, reinterpret_cast<const uint8_t*>(dbSeqBuf), offsets_.data(),
                       lengths_.data(), lengths_.size(), &dopt, &dbs_[slot]) != SWH_OK)
         throw swh_fail("swh_db_create");
     scores_[slot].resize(lengths_.size());
-    e->setEntry(dbSeqBuf, nullptr, scores_[slot].data());
+    e->setEntry(dbSeqBuf, NULL, scores_[slot].data());
     e->setBaseIdx(fillBaseIndex);
     e->setBatchNum((int)infoSize);
-    e->setResultSize(lengths_.size());
+    e->setResultSize((int)lengths_.size());
 }
 
 void HIPWorker::search() {
     const int slot = (int)bufFlag;
-    swh_stats st{};
-    if (swh_search(ctx_, dbs_[slot], reinterpret_cast<const uint8_t*>(query.data()), queryLen,
-                   nullptr, compEntry->getScores(), &st) != SWH_OK)
+    swh_stats st = {};
+    if (swh_search(ctx_, dbs_[slot], reinterpret_cast<const uint8_t*>(query), queryLen, NULL,
+                   compEntry->getScores(), &st) != SWH_OK)
         throw swh_fail("swh_search");
     kernelMs_ += st.kernel_ms + st.rescore_ms;
     cells_ += st.cells;
-    if (sink_) {
-        const int* s = compEntry->getScores();
-        for (size_t i = 0; i < compEntry->getResultSize(); ++i)
-            sink_[compEntry->getBaseIdx() + i] = s[i];
-    }
 }
 
 void HIPWorker::showPerformance() {
     const double t = getStopTime() - getStartTime();
     std::fprintf(stderr, "HIP[%d] computing time %lfs, AARes %zu, GCUPs %lf\n", deviceId_, t,
-                TotalAminoAcidResidue, t > 0 ? TotalAminoAcidResidue * (double)queryLen / t / 1e9 : 0.0);
+                 (size_t)TotalAminoAcidResidue,
+                 t > 0 ? TotalAminoAcidResidue * (double)queryLen / t / 1e9 : 0.0);
 }
 
 void HIPWorker::who() { std::fprintf(stderr, "HIP\n"); }

@@ -28,7 +28,8 @@ char* Worker::allocChunk(size_t bytes) { return (char*)std::malloc(bytes); }
 void Worker::freeChunk(char* p) { std::free(p); }
 
 void Worker::setQuery(const char* q, size_t qlen) {
-    query.assign(q, q + qlen);
+    queryStore_.assign(q, q + qlen);
+    query = queryStore_.data();
     queryLen = (int)qlen;
 }
 

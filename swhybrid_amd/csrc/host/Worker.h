@@ -81,8 +81,9 @@ class Worker {
   protected:
     virtual void who() {}
     virtual void packDB();
-    // the chunk buffer the DataPool fills (HIPWorker: page-locked memory);
-    // a subclass that overrides these frees dbSeqBuf in its own destructor
+    // the chunk buffer the DataPool fills, allocated at the first fill with
+    // bufferSize bytes (PoolHIPWorker: page-locked memory); a subclass that
+    // overrides these frees dbSeqBuf in its own destructor
     virtual char* allocChunk(size_t bytes);
     virtual void freeChunk(char* p);
     bool fill();
@@ -100,8 +101,9 @@ class Worker {
             .count();
     }
 
-    // query and scoring
-    std::vector<char> query;
+    // query and scoring: a subclass's setQuery points query at its own copy
+    // (as the reference's workers do); Worker::setQuery keeps one in the base
+    char* query = nullptr;
     int queryLen = 0;
     int matrix[32][32];
     int topNum;
@@ -134,6 +136,7 @@ class Worker {
     int* sink_ = nullptr;
 
   private:
+    std::vector<char> queryStore_;   // Worker::setQuery's copy of the query
     static void* threadEntry(void* self);
     static void* auxEntryFunc(void* self);
     void threadControl();

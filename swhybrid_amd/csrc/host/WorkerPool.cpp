@@ -8,7 +8,7 @@ WorkerPool::WorkerPool(DataPool* loader, Param* params, size_t chunkBytes, size_
     recalcWorker_ = new RecalcWorker(params, 0);
     recalcWorker_->setQuery(loader->getQuery(), loader->getQueryLen());
     for (int i = 0; i < params->hipNum; ++i) {
-        workerList_.push_back(new HIPWorker(loader, params, recalcWorker_, i, chunkBytes));
+        workerList_.push_back(new PoolHIPWorker(loader, params, recalcWorker_, i, chunkBytes));
         isCpu_.push_back(false);
     }
     if (params->cpuNum > 0) {

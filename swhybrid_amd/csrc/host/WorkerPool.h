@@ -14,14 +14,14 @@
 
 #include "CPUWorker.h"
 #include "DataPool.h"
-#include "HIPWorker.h"
+#include "PoolHIPWorker.h"
 #include "Param.h"
 #include "RecalcWorker.h"
 #include "ResultList.h"
 
 class WorkerPool {
   public:
-    // params->hipNum HIPWorkers (chunks of chunkBytes) and, when
+    // params->hipNum PoolHIPWorkers (chunks of chunkBytes) and, when
     // params->cpuNum > 0, one CPUWorker with that many threads (chunks of
     // cpuChunkBytes), in that order (the reference creates the CPU worker
     // first, src/WorkerPool.cpp:47-52; the order only decides ties).
