@@ -13,10 +13,14 @@ Prints ONE JSON line (metric GCUPS, higher is better).  Extra keys:
                 timed launches / their HIP-event time) over 256 CUs x 64
                 lane-instructions x 2.4 GHz; beside it the 9-op rate of the
                 reference's recurrence and the HBM side, as flat fields
-  cpu_baseline  the reference's own AVX2 kernels (oracle/_ref, built from the
-                reference sources) on every 3rd sequence of the database (about
-                10 s of work), timed on this host's cores; the oracle's C
-                restatement if _ref is absent
+  cpu_baseline  this build's restatement of the reference's CPU path (the
+                host AVX2 scan of libswhybrid_host.so: int8 pass, then int16
+                for lanes that may have saturated, over a packed database) on
+                every 3rd sequence of the database with all five queries,
+                timed on this host's cores (its CPU share: OMP_NUM_THREADS
+                where set), with the host's model and core counts and the
+                scan's committed calibration against the reference's own
+                kernels
   per_query     GCUPS and kernel time for each query length
   batched, batched_small_db, dbfile, hybrid, served, overflow_stress
                 the other workloads (64-query batch, small database, database
@@ -53,35 +57,100 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def host_cpu():
+    """Model name and core counts of this host."""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"model": model, "nproc": os.cpu_count(),
+            "affinity": len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None}
+
+
+def cpu_threads():
+    """This process's CPU share: OMP_NUM_THREADS where the host sets it, else
+    every core it may run on."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        return int(env)
+    return host_cpu()["affinity"] or os.cpu_count() or 1
+
+
+def cpu_calibration():
+    path = os.path.join(ROOT, "profiles", "cpu_baseline_calibration.json")
+    try:
+        with open(path) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return None
+
+
 def cpu_baseline(db, queries, matrix, go, ge, stride=3):
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle as O
+    """The reference's CPU path restated (the host AVX2 scan, int8 then int16)
+    over one sequence in `stride` of the same database, every query, on this
+    host's cores; the database is packed first (untimed, as the GPU leg's is
+    resident)."""
     from swhybrid_amd.datagen import SeqDB
+    from swhybrid_amd.engine import cpu_search
     idx = np.arange(0, db.n, stride)
     sub = SeqDB(db.residues, db.offsets[idx].copy(), db.lengths[idx].copy())
-    threads = min(16, os.cpu_count() or 1)
+    threads = cpu_threads()
     cells = 0
-    t = 0.0
-    if O.ref_available():
-        kind = "reference"
-        for q in queries:
-            t0 = time.perf_counter()
-            O.ref_search(q, sub, matrix, go, ge, path="avx2", nthreads=threads)
-            t += time.perf_counter() - t0
-            cells += sub.total * len(q)
-    else:
-        kind = "port"
-        for q in queries[:1]:
-            t0 = time.perf_counter()
-            O.search(q, sub, matrix, go, ge, nthreads=threads)
-            t += time.perf_counter() - t0
-            cells += sub.total * len(q)
-    return {
-        "value": round(cells / t / 1e9, 2), "unit": "GCUPS", "cores": threads, "kind": kind,
+    t = pack = 0.0
+    for q in queries:
+        scores, secs, psecs = cpu_search(sub, q, matrix=matrix, gap_open=go, gap_extend=ge,
+                                         threads=threads)
+        if (scores < 0).any():
+            raise RuntimeError("cpu baseline: a score beyond int16 on this workload")
+        t += secs
+        pack += psecs
+        cells += sub.total * len(q)
+    out = {
+        "value": round(cells / t / 1e9, 2), "unit": "GCUPS", "cores": threads, "kind": "port",
         "sample": (f"one sequence in {stride} of the same database ({sub.n} sequences, "
                    f"{sub.total} residues) x query lengths {[len(q) for q in queries]}; "
-                   f"8-bit pass + int16 rescore; {t:.1f} s"),
+                   f"this build's AVX2 restatement of the reference's avxCompute + "
+                   f"avxCompute16 (int8 pass, int16 for saturated lanes) over the packed "
+                   f"sample; {t:.1f} s scan, {pack:.1f} s packing (not in value)"),
+        "host": host_cpu(),
     }
+    cal = cpu_calibration()
+    if cal:
+        out["calibration"] = cal
+    return out
+
+
+def cpu_sweep():
+    """Query segmentation of the host scan: GCUPS per segment length at query
+    lengths 5478 and 9000 over 30k sequences of the 5M database's length
+    distribution, on this host's cores and on one."""
+    from swhybrid_amd import datagen as G
+    from swhybrid_amd.engine import cpu_search
+    db = G.c2_db(n=30_000)
+    res = {"host": host_cpu(), "workload": f"{db.n} sequences, {db.total} residues",
+           "gcups": {}}
+    for threads in sorted({1, cpu_threads()}):
+        for L in (5478, 9000):
+            q = G.queries([L], 12)[0]
+            ref = None
+            for seg in (-1, 4096, 2048, 1024, 512):
+                best = 1e30
+                for _ in range(2):
+                    got, secs, _ = cpu_search(db, q, threads=threads, query_segment=seg)
+                    best = min(best, secs)
+                if ref is None:
+                    ref = got
+                elif not np.array_equal(got, ref):
+                    raise RuntimeError(f"segment {seg} changed a score at query length {L}")
+                key = f"threads={threads} qlen={L} segment={'whole' if seg < 0 else seg}"
+                res["gcups"][key] = round(db.total * L / best / 1e9, 1)
+                log(f"{key}: {res['gcups'][key]} GCUPS")
+    print(json.dumps(res))
 
 
 # Algorithmic bytes per k_scan launch of this workload: the packed database
@@ -393,9 +462,14 @@ def main():
                     help="skip the 2M-sequence near-copy workload")
     ap.add_argument("--serve-qps", type=str, default="5,20,50")
     ap.add_argument("--serve-requests", type=int, default=60)
+    ap.add_argument("--cpu-sweep", action="store_true",
+                    help="only measure the host scan's query segmentation and print it")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
+    if args.cpu_sweep:
+        cpu_sweep()
+        return
     if args.copy_calibration:
         import torch
         n = args.copy_calibration << 20

@@ -36,6 +36,24 @@ int swhy_pool_search(const uint8_t* residues, const uint64_t* offsets, const uin
                      int64_t* out_top_pool_index, int32_t* out_scores, double* out_seconds,
                      double* out_gcups);
 
+/* The host's AVX2 scan alone, over a whole database, with a team of nthreads
+ * threads (0 = every core): the CPU worker's kernel without the pipeline.
+ * Sequences longest first, 32 per 256-bit vector in biased int8, the lanes
+ * that may have saturated again 16 per vector in biased int16; a score at or
+ * above 65536 - max(matrix) comes back as -1 (the reference's int16 pass is
+ * exact only below that too).  query_segment: query columns per pass of the
+ * scan (0 = default, -1 = whole query).  The database is first packed, 32
+ * sequences per group interleaved row by row (out_pack_seconds, as the
+ * reference's packDB prepares a chunk before its search), then scanned
+ * (out_seconds).
+ * Counterpart: avxCompute under the reference's OpenMP team with the
+ * avxCompute16 rescore of its overflowed lanes (src/AVXKernel.cpp:408-554,
+ * 866-1016; src/AVXWorker.cpp:397-427; src/RecalcWorker.cpp:184-257). */
+int swhy_cpu_search(const uint8_t* residues, const uint64_t* offsets, const uint32_t* lengths,
+                    uint64_t nseq, const uint8_t* query, int qlen, const int32_t* matrix,
+                    int gap_open, int gap_extend, int nthreads, int query_segment,
+                    int32_t* out_scores, double* out_seconds, double* out_pack_seconds);
+
 /* ---- Hybrid pool: HIPWorkers and a CPU worker on one DataPool ----------------
  * The reference's hybrid scheduling (src/WorkerPool.cpp:25-78): every worker
  * pulls chunks from the same DataPool, so the faster device takes more of

@@ -174,7 +174,7 @@ HOST_EXPORTS = ("swhy_last_error", "swhy_pool_search", "swhy_dbfile_make", "swhy
                 "swhy_dbfile_info", "swhy_dbfile_arrays", "swhy_dbfile_title",
                 "swhy_dbfile_close", "swhy_dbfile_pool_search", "swhy_fasta_open",
                 "swhy_fasta_count", "swhy_fasta_record", "swhy_fasta_close",
-                "swhy_pool_search_ex", "swhy_dbfile_pool_search_ex")
+                "swhy_pool_search_ex", "swhy_dbfile_pool_search_ex", "swhy_cpu_search")
 
 
 class SwhyPoolOpts(C.Structure):
@@ -230,6 +230,9 @@ def host_lib():
     L.swhy_dbfile_pool_search_ex.argtypes = [vp, vp, C.c_int, vp, C.c_int, C.c_int,
                                              C.POINTER(SwhyPoolOpts), vp, vp, vp,
                                              C.POINTER(SwhyPoolReport)]
+    L.swhy_cpu_search.argtypes = [vp, vp, vp, C.c_uint64, vp, C.c_int, vp, C.c_int, C.c_int,
+                                  C.c_int, C.c_int, vp, C.POINTER(C.c_double),
+                                  C.POINTER(C.c_double)]
     for name in HOST_EXPORTS[1:]:
         getattr(L, name).restype = C.c_int
     _host = L

@@ -59,9 +59,9 @@ void CPUWorker::search() {
     const std::vector<batchInfo>& bs = batches_[slot];
     const uint8_t* data = reinterpret_cast<const uint8_t*>(buf_[slot].data());
     int* out = scores_[slot].data();
-    // every sequence of the chunk, longest first, cut into pieces of whole
-    // 32-lane groups of about 256 KB of residues: batches of one length can
-    // be small (a long tail of distinct lengths), so groups span batches
+    // every sequence of the chunk; the team takes them longest first in
+    // pieces of whole 32-lane groups (batches of one length can be small, a
+    // long tail of distinct lengths, so groups span batches)
     std::vector<const uint8_t*> ptr;
     std::vector<uint32_t> len;
     size_t boff = 0;
@@ -76,60 +76,16 @@ void CPUWorker::search() {
         boff += (size_t)b.numSeqs * b.seqLen;
     }
     const size_t nseq = ptr.size();
-    std::vector<uint32_t> order(nseq);
-    for (size_t i = 0; i < nseq; ++i) order[i] = (uint32_t)i;
-    std::stable_sort(order.begin(), order.end(),
-                     [&](uint32_t a, uint32_t b) { return len[a] > len[b]; });
-    struct Piece {
-        size_t first, n;
-    };
-    std::vector<Piece> pieces;
-    for (size_t i = 0; i < nseq;) {
-        size_t n = 0, bytes = 0;
-        while (i + n < nseq && (n == 0 || n % 32 != 0 || bytes < (size_t(256) << 10))) {
-            bytes += len[order[i + n]];
-            ++n;
-        }
-        pieces.push_back(Piece{i, n});
-        i += n;
-    }
-    const uint8_t* q = reinterpret_cast<const uint8_t*>(query);
+    swhy::cpu::score_team(scoring_, reinterpret_cast<const uint8_t*>(query), queryLen, ptr.data(),
+                          len.data(), nseq, out, threads_);
     const size_t base = compEntry->getBaseIdx();
-    std::atomic<size_t> next{0};
-    std::atomic<uint64_t> pushed{0};
-    auto work = [&]() {
-        swhy::cpu::Scratch scratch;
-        std::vector<int32_t> sc;
-        std::vector<const uint8_t*> pp;
-        std::vector<uint32_t> pl;
-        for (size_t p = next++; p < pieces.size(); p = next++) {
-            const Piece& pc = pieces[p];
-            pp.resize(pc.n);
-            pl.resize(pc.n);
-            for (size_t k = 0; k < pc.n; ++k) {
-                pp[k] = ptr[order[pc.first + k]];
-                pl[k] = len[order[pc.first + k]];
-            }
-            sc.resize(pc.n);
-            swhy::cpu::score_seqs(scoring_, q, queryLen, pp.data(), pl.data(), (uint32_t)pc.n,
-                                  sc.data(), scratch);
-            for (size_t k = 0; k < pc.n; ++k) {
-                const uint32_t i = order[pc.first + k];
-                int v = sc[k];
-                if (v < 0) {   // beyond int16: exact int32 after the workers finish
-                    recalcWorker->pushOverflowSeq(reinterpret_cast<const char*>(ptr[i]), len[i],
-                                                  base + i);
-                    ++pushed;
-                    v = 0;
-                }
-                out[i] = v;
-            }
+    uint64_t pushed = 0;
+    for (size_t i = 0; i < nseq; ++i)
+        if (out[i] < 0) {   // beyond int16: exact int32 after the workers finish
+            recalcWorker->pushOverflowSeq(reinterpret_cast<const char*>(ptr[i]), len[i], base + i);
+            ++pushed;
+            out[i] = 0;
         }
-    };
-    std::vector<std::thread> team;
-    for (int t = 1; t < threads_; ++t) team.emplace_back(work);
-    work();
-    for (auto& t : team) t.join();
     pushed_ += pushed;
     if (sink_)
         for (size_t i = 0; i < compEntry->getResultSize(); ++i) sink_[base + i] = out[i];

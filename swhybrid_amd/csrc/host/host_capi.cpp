@@ -9,8 +9,10 @@
 #include <exception>
 #include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
+#include "CpuKernel.h"
 #include "DataPool.h"
 #include "DbFile.h"
 #include "Param.h"
@@ -93,6 +95,49 @@ struct swhy_fasta {
 extern "C" {
 
 const char* swhy_last_error(void) { return g_err.c_str(); }
+
+int swhy_cpu_search(const uint8_t* residues, const uint64_t* offsets, const uint32_t* lengths,
+                    uint64_t nseq, const uint8_t* query, int qlen, const int32_t* matrix,
+                    int gap_open, int gap_extend, int nthreads, int query_segment,
+                    int32_t* out_scores, double* out_seconds, double* out_pack_seconds) {
+    if ((nseq > 0 && (!residues || !offsets || !lengths || !out_scores)) || !query || qlen <= 0 ||
+        !matrix || nthreads < 0 || query_segment < -1 || gap_open < 0 || gap_extend < 0) {
+        g_err = "swhy_cpu_search: bad arguments";
+        return -1;
+    }
+    for (int j = 0; j < qlen; ++j)
+        if (query[j] > 23) {
+            g_err = "swhy_cpu_search: query code above 23";
+            return -1;
+        }
+    for (int i = 0; i < 1024; ++i)
+        if (matrix[i] < -128 || matrix[i] > 127) {
+            g_err = "swhy_cpu_search: matrix entry outside [-128, 127]";
+            return -1;
+        }
+    if (!swhy::cpu::avx2_available()) {
+        g_err = "swhy_cpu_search: the host has no AVX2";
+        return -3;
+    }
+    swhy::cpu::Scoring sc;
+    std::memcpy(sc.matrix, matrix, sizeof(sc.matrix));
+    sc.gap_open = gap_open;
+    sc.gap_extend = gap_extend;
+    sc.query_segment = query_segment;
+    if (nthreads == 0) nthreads = (int)std::max(1u, std::thread::hardware_concurrency());
+    std::vector<const uint8_t*> ptr(nseq);
+    for (uint64_t i = 0; i < nseq; ++i) ptr[i] = residues + offsets[i];
+    using clock = std::chrono::steady_clock;
+    const auto t0 = clock::now();
+    swhy::cpu::PackedDb db;
+    swhy::cpu::pack_db(ptr.data(), lengths, nseq, db, nthreads);
+    const auto t1 = clock::now();
+    swhy::cpu::scan_db(sc, query, qlen, db, out_scores, nthreads);
+    const auto t2 = clock::now();
+    if (out_pack_seconds) *out_pack_seconds = std::chrono::duration<double>(t1 - t0).count();
+    if (out_seconds) *out_seconds = std::chrono::duration<double>(t2 - t1).count();
+    return 0;
+}
 
 int swhy_pool_search(const uint8_t* residues, const uint64_t* offsets, const uint32_t* lengths,
                      uint64_t nseq, const uint8_t* query, int qlen, const int32_t* matrix,

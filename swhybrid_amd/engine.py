@@ -293,6 +293,29 @@ def pool_search(db: SeqDB, query, matrix="blosum62", gap_open=10, gap_extend=2, 
             "seconds": secs.value, "gcups": gc.value}
 
 
+def cpu_search(db: SeqDB, query, matrix="blosum62", gap_open=10, gap_extend=2, threads=0,
+               query_segment=0):
+    """The host's AVX2 scan alone (int8 pass, int16 for lanes that may have
+    saturated) over the whole database with `threads` threads (0 = every
+    core).  Returns (scores, scan seconds, pack seconds); a score beyond
+    int16 is -1."""
+    m = _as_matrix(matrix)
+    q = _as_query(query)
+    res = np.ascontiguousarray(db.residues, dtype=np.uint8)
+    off = np.ascontiguousarray(db.offsets, dtype=np.uint64)
+    lens = np.ascontiguousarray(db.lengths, dtype=np.uint32)
+    out = np.zeros(db.n, dtype=np.int32)
+    secs, pack = C.c_double(0), C.c_double(0)
+    L = N.host_lib()
+    rc = L.swhy_cpu_search(N.ptr(res), N.ptr(off), N.ptr(lens), db.n, N.ptr(q), len(q), N.ptr(m),
+                           int(gap_open), int(gap_extend), int(threads), int(query_segment),
+                           N.ptr(out), C.byref(secs), C.byref(pack))
+    if rc != 0:
+        msg = L.swhy_last_error()
+        raise N.SwhError(f"cpu search failed ({rc}): {msg.decode() if msg else ''}")
+    return out, secs.value, pack.value
+
+
 def hybrid_search(db: SeqDB, query, matrix="blosum62", gap_open=10, gap_extend=2, k=10,
                   hip_workers=1, cpu_threads=0, chunk_bytes=0, cpu_chunk_bytes=0,
                   all_scores=False):
