@@ -393,17 +393,28 @@ def run_hybrid(db, qlens=(1000, 5478)):
     return out
 
 
-def run_serving(eng, d, qps_levels, n_requests):
+def run_serving(eng, d, qps_levels, n_requests, round_ms=0.0, slices=0, wait_weight=0.0):
     """Poisson arrivals at fixed QPS, lengths logN(mean 400, sigma 0.7) clipped
     to [30, 5000], top-20 per request, the database resident; p50/p99 latency
-    (submit to top-k delivered) and sustained GCUPS per level."""
+    (submit to top-k delivered) and sustained GCUPS per level.  Each request
+    is also timed alone first (one search call, the device otherwise idle):
+    no scheduler can deliver it faster, so `slowdown` = latency / alone time
+    is what the serving loop adds (waiting and sharing the device)."""
     import threading
     from swhybrid_amd import datagen as G
     from swhybrid_amd.engine import Server
     res = {}
+    alone_ms = {}
     for qps in qps_levels:
         arrivals, qs = G.c4_requests(n_requests, qps)
-        srv = Server(eng, d)
+        for q in qs:
+            key = q.tobytes()
+            if key not in alone_ms:
+                eng.search(d, q)
+                t0 = time.perf_counter()
+                eng.search(d, q)
+                alone_ms[key] = (time.perf_counter() - t0) * 1e3
+        srv = Server(eng, d, round_ms=round_ms, slices=slices, wait_weight=wait_weight)
         ids = [None] * len(qs)
         t0 = time.perf_counter()
 
@@ -417,6 +428,7 @@ def run_serving(eng, d, qps_levels, n_requests):
         th = threading.Thread(target=feed)
         th.start()
         lat, done_ms, first_submit = [], [], None
+        slow, rounds = [], []
         cells = 0
         for i in range(len(qs)):
             while ids[i] is None:
@@ -424,6 +436,8 @@ def run_serving(eng, d, qps_levels, n_requests):
             r = srv.wait(ids[i])
             info = r["info"]
             lat.append(info["latency_ms"])
+            slow.append(info["latency_ms"] / alone_ms[qs[i].tobytes()])
+            rounds.append(info["rounds"])
             done_ms.append(info["done_ms"])
             first_submit = info["submit_ms"] if first_submit is None else min(first_submit, info["submit_ms"])
             cells += info["cells"]
@@ -432,15 +446,24 @@ def run_serving(eng, d, qps_levels, n_requests):
         srv.close()
         span = (max(done_ms) - first_submit) / 1e3
         lat = np.array(lat)
+        slow = np.array(slow)
+        alone = np.array([alone_ms[q.tobytes()] for q in qs])
         res[str(int(qps)) if float(qps).is_integer() else str(qps)] = {
             "requests": len(qs), "p50_ms": round(float(np.percentile(lat, 50)), 1),
             "p99_ms": round(float(np.percentile(lat, 99)), 1),
             "mean_ms": round(float(lat.mean()), 1),
+            "alone_p50_ms": round(float(np.percentile(alone, 50)), 1),
+            "alone_p99_ms": round(float(np.percentile(alone, 99)), 1),
+            "slowdown_p50": round(float(np.percentile(slow, 50)), 2),
+            "slowdown_p99": round(float(np.percentile(slow, 99)), 2),
             "gcups": round(cells / span / 1e9, 1), "offered_tcups": round(
                 qps * np.mean([len(q) for q in qs]) * d.residues / 1e12, 2),
-            "peak_inflight": st["peak_inflight"], "rounds": st["rounds"]}
-        log(f"serving {qps} QPS: p50 {res[list(res)[-1]]['p50_ms']} ms, "
-            f"p99 {res[list(res)[-1]]['p99_ms']} ms")
+            "peak_inflight": st["peak_inflight"], "rounds": st["rounds"],
+            "max_rounds_per_request": int(max(rounds))}
+        r_ = res[list(res)[-1]]
+        log(f"serving {qps} QPS: p50 {r_['p50_ms']} ms, p99 {r_['p99_ms']} ms (alone "
+            f"{r_['alone_p50_ms']} / {r_['alone_p99_ms']}), slowdown p50 {r_['slowdown_p50']} "
+            f"p99 {r_['slowdown_p99']}, {r_['gcups']} GCUPS, {r_['rounds']} rounds")
     return res
 
 
@@ -462,6 +485,12 @@ def main():
                     help="skip the 2M-sequence near-copy workload")
     ap.add_argument("--serve-qps", type=str, default="5,20,50")
     ap.add_argument("--serve-requests", type=int, default=60)
+    ap.add_argument("--serve-round-ms", type=float, default=0.0,
+                    help="device work per serving round in ms (0 = the library default)")
+    ap.add_argument("--serve-wait-weight", type=float, default=0.0,
+                    help="priority credit of waiting (0 = the library default)")
+    ap.add_argument("--serve-slices", type=int, default=0,
+                    help="0 = budgeted rounds, n = every request in n fixed slices")
     ap.add_argument("--cpu-sweep", action="store_true",
                     help="only measure the host scan's query segmentation and print it")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -595,7 +624,8 @@ def main():
         out["hybrid"] = run_hybrid(db)
     if not args.no_serve:
         out["served"] = run_serving(eng, d, [float(x) for x in args.serve_qps.split(",")],
-                                    args.serve_requests)
+                                    args.serve_requests, args.serve_round_ms,
+                                    args.serve_slices, args.serve_wait_weight)
     d.close()
     eng.close()
     if not args.no_overflow_stress:

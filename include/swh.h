@@ -237,13 +237,23 @@ typedef struct swh_server swh_server;
 typedef struct swh_server_opts {
     int max_inflight;          /* requests on the device at once (0 = 64) */
     uint64_t memory_budget;    /* device bytes for request buffers (0 = 4 GiB) */
-    int slices;                /* database slices per request, each a round that new
-                                  requests can join (0 = 1: on MI355X every slice
-                                  adds a launch tail; one slice gave the lowest
-                                  p50 and p99 at a sustainable load) */
+    int slices;                /* 0 = budgeted rounds: each round holds about
+                                  round_ms of device work, requests served
+                                  shortest remaining work first (with aging), a
+                                  long request advancing one slice per round;
+                                  n >= 1 = every request in n fixed slices, each a
+                                  round (1: the whole database per round).  A
+                                  small database (two layouts) always runs whole
+                                  rounds */
     int max_qlen;              /* longest admissible query (0 = 8192) */
     int max_k;                 /* largest top-k a request may ask for (0 = 100) */
     swh_search_opts search;    /* strip widths, overflow threshold */
+    double round_ms;           /* budgeted rounds: device work per round in ms
+                                  (0 = 160) */
+    double wait_weight;        /* budgeted rounds: a request's priority is its
+                                  remaining work less wait_weight x the work the
+                                  device could have done while it waited (0 =
+                                  0.5; larger favours older requests) */
 } swh_server_opts;
 
 typedef struct swh_request_info {

@@ -68,6 +68,8 @@ class SwhServerOpts(C.Structure):
         ("max_qlen", C.c_int),
         ("max_k", C.c_int),
         ("search", SwhSearchOpts),
+        ("round_ms", C.c_double),
+        ("wait_weight", C.c_double),
     ]
 
 

@@ -652,6 +652,15 @@ int enqueue_tasks(swh_ctx* c, swh_db* db, const Task* t, int nt, const Shape& sh
                   swh_stats* st) {
     // exact below both the saturation of an add (65535 - max(matrix)) and the
     // fp16 ordering of the three-way max
+    // a range outside its block kind would make the scan read pair blocks as
+    // lane blocks (or the reverse), past their rows' border slots
+    for (int i = 0; i < nt; ++i)
+        if (t[i].pair_begin < 0 || t[i].pair_begin > t[i].pair_end || t[i].pair_end > db->n_pairs ||
+            t[i].lane_begin < db->n_pairs || t[i].lane_begin > t[i].lane_end ||
+            t[i].lane_end > db->n_blocks)
+            return fail(SWH_ERR_ARG, "task %d: pair blocks [%d, %d) / lane blocks [%d, %d) outside "
+                        "[0, %d) / [%d, %d)", i, t[i].pair_begin, t[i].pair_end, t[i].lane_begin,
+                        t[i].lane_end, db->n_pairs, db->n_pairs, db->n_blocks);
     const int exact_thr = std::min(65535 - c->max_score, kScoreLimitF16);
     const int thr = (o.overflow_threshold > 0) ? std::min(o.overflow_threshold, exact_thr)
                                                : exact_thr;

@@ -3,11 +3,20 @@
 // A loop thread repeats rounds.  Each round it admits waiting requests (in
 // arrival order) while their device buffers fit the memory budget and a slot
 // is free, then enqueues ONE scan launch (plus the rescore) covering the next
-// database slice of every admitted request, waits for it, and hands the
-// requests that finished their last slice to a completion thread (top-k on the
-// host from the pinned copy of the score vector).  Scores of a request depend
-// only on its query and the database: each task scans its own block ranges and
+// database slice of admitted requests, waits for it, and hands the requests
+// that finished their last slice to a completion thread (top-k on the host
+// from the pinned copy of the score vector).  Scores of a request depend only
+// on its query and the database: each task scans its own block ranges and
 // writes its own buffers, whatever else shares the launch.
+//
+// Slices.  By default a round holds about round_ms of device work: each
+// request keeps a cursor over the database's pair and lane blocks, requests
+// are served shortest remaining work first (with aging, so none waits
+// forever), and each one's slice covers the same share of its remaining pair
+// and lane work.  A short request that arrives while a long one is on the
+// device therefore runs in the next round instead of after the long one; a
+// long one advances a slice per round.  With opts.slices = n every request is
+// cut into n fixed slices instead (n = 1: the whole database per round).
 #include <condition_variable>
 #include <deque>
 #include <memory>
@@ -15,6 +24,7 @@
 #include <unordered_map>
 
 #include "swh_host.h"
+#include "swh_plan.h"
 
 using namespace swhh;
 
@@ -26,7 +36,10 @@ struct Request {
     int k = 0;
     bool want_scores = false;
     int slot = -1;
-    int next_slice = 0;
+    int next_slice = 0;                // fixed slices: the next one to run
+    // budgeted rounds: the next block of each kind (lane blocks start after
+    // the pair blocks; swh_server_submit sets lane_pos to the first of them)
+    int pair_pos = 0, lane_pos = 0;
     size_t bytes = 0;                 // device bytes charged against the budget
     swh_request_info info{};
     std::vector<int32_t> top_score;
@@ -45,6 +58,8 @@ struct swh_server {
     // slices of the block ranges
     std::vector<int> pair_cut, lane_cut;
     size_t max_slice_seqs = 0;
+    bool budgeted = false;       // rounds of about round_ms of work (else fixed slices)
+    double rate = 9.0e9;         // padded cells per ms the rounds have run at (estimate)
     // per-slot device buffers and pinned score copies
     DevBuf scores, ovf, prof, counters;
     PinnedBuf h_prof, h_scores;
@@ -69,6 +84,18 @@ struct swh_server {
 
 namespace {
 
+// Device work per budgeted round: a request that arrives while others are on
+// the device waits at most about this long to join them.
+// 160 ms: on the 5M-sequence database with Poisson requests of log-normal
+// length (mean 400), 40 / 80 / 160 ms rounds against whole-request rounds gave
+// p50 / p99 of 73 / 377, 78 / 324, 77 / 316 against 76 / 266 ms at 5 QPS,
+// 1,316 / 1,933, 1,229 / 1,780, 1,201 / 1,761 against 1,558 / 2,354 ms at 20
+// QPS, and 8,545, 9,031, 9,218 against 9,233 GCUPS at 50 QPS: a round costs a
+// few ms of launch tail and host work, and a lane item of the longest lane
+// blocks of a long query runs for longer than a short round anyway.
+constexpr double kDefaultRoundMs = 160.0;
+constexpr double kDefaultWaitWeight = 0.5;
+
 double since(const swh_server* s) { return now_ms() - s->t0; }
 
 size_t request_bytes(const swh_server* s, int qlen) {
@@ -89,6 +116,26 @@ std::vector<int> cut_blocks(const swh_db* db, int b0, int b1, int n) {
         cut[i] = std::max(b, cut[i - 1]);
     }
     return cut;
+}
+
+// Budgeted round: the slices of swh_plan.h's plan_budgeted, priced at the
+// rate the rounds have run at, with two lane items per workgroup slot of a
+// full grid (2 workgroups per CU, 4 lane blocks per item) per round.
+template <typename AddTask>
+void plan_round(swh_server* s, const std::vector<std::shared_ptr<Request>>& batch,
+                AddTask&& add_task) {
+    const swh_db* db = s->db;
+    const double now = since(s);
+    std::vector<PlanRequest> reqs;
+    for (const auto& rp : batch)
+        reqs.push_back(PlanRequest{(int)rp->query.size(), rp->pair_pos, rp->lane_pos,
+                                   now - rp->info.submit_ms});
+    const std::vector<PlanSlice> plan =
+        plan_budgeted(db->blk_padded.data(), db->n_pairs, db->n_blocks, reqs,
+                      s->opts.round_ms * s->rate, s->rate, 16 * std::max(s->ctx->num_cus, 1),
+                      s->opts.wait_weight);
+    for (const PlanSlice& p : plan)
+        add_task(batch[p.request], p.pair_begin, p.pair_end, p.lane_begin, p.lane_end, p.last);
 }
 
 // What a round ended with: work done, a clean stop (nothing active, the
@@ -138,31 +185,43 @@ Round run_round(swh_server* s) {
             return Round::kFail;
         }
     }
-    std::vector<Task> tasks(batch.size());
-    std::vector<int> qlens(batch.size());
     const size_t nseq = db->nseq;
     // whole-database rounds of a small database run on the layout that suits
     // the round's request count (slices are block ranges of the first layout)
-    if (s->opts.slices == 1) db = pick_layout(c, db, (int)batch.size());
-    for (size_t i = 0; i < batch.size(); ++i) {
-        Request& r = *batch[i];
-        Task& T = tasks[i];
+    if (s->opts.slices == 1 && !s->budgeted) db = pick_layout(c, db, (int)batch.size());
+    std::vector<Task> tasks;
+    std::vector<int> qlens;
+    std::vector<std::shared_ptr<Request>> ran;   // the requests of this round, task order
+    std::vector<bool> last;                      // ... and whether it is their last slice
+    auto add_task = [&](const std::shared_ptr<Request>& rp, int pb, int pe, int lb, int le,
+                        bool is_last) {
+        Request& r = *rp;
+        Task T{};
         T.prof = (const int16_t*)s->prof.p + (size_t)r.slot * s->prof_slot_elems;
         T.qlen = (int)r.query.size();
         T.scores = (int32_t*)s->scores.p + (size_t)r.slot * nseq;
         T.ovf = (uint32_t*)s->ovf.p + (size_t)r.slot * s->max_slice_seqs;
-        if (s->opts.slices == 1) {
-            T.pair_begin = 0;
-            T.pair_end = db->n_pairs;
-            T.lane_begin = db->n_pairs;
-            T.lane_end = db->n_blocks;
-        } else {
-            T.pair_begin = s->pair_cut[r.next_slice];
-            T.pair_end = s->pair_cut[r.next_slice + 1];
-            T.lane_begin = s->lane_cut[r.next_slice];
-            T.lane_end = s->lane_cut[r.next_slice + 1];
+        T.pair_begin = pb;
+        T.pair_end = pe;
+        T.lane_begin = lb;
+        T.lane_end = le;
+        tasks.push_back(T);
+        qlens.push_back(T.qlen);
+        ran.push_back(rp);
+        last.push_back(is_last);
+    };
+    if (!s->budgeted) {
+        for (auto& rp : batch) {
+            const Request& r = *rp;
+            if (s->opts.slices == 1)
+                add_task(rp, 0, db->n_pairs, db->n_pairs, db->n_blocks, true);
+            else
+                add_task(rp, s->pair_cut[r.next_slice], s->pair_cut[r.next_slice + 1],
+                         s->lane_cut[r.next_slice], s->lane_cut[r.next_slice + 1],
+                         r.next_slice + 1 == s->opts.slices);
         }
-        qlens[i] = T.qlen;
+    } else {
+        plan_round(s, batch, add_task);
     }
     const Shape sh = choose_shape(
         c, qlens.data(), (int)qlens.size(), s->opts.search, db,
@@ -177,11 +236,12 @@ Round run_round(swh_server* s) {
     }
     // requests on their last slice: copy the scores out
     std::vector<std::shared_ptr<Request>> finished;
-    for (auto& r : batch)
-        if (r->next_slice + 1 == s->opts.slices) {
-            finished.push_back(r);
-            if (hipMemcpyAsync((int32_t*)s->h_scores.p + (size_t)r->slot * nseq,
-                               (int32_t*)s->scores.p + (size_t)r->slot * nseq,
+    for (size_t i = 0; i < ran.size(); ++i)
+        if (last[i]) {
+            const Request& r = *ran[i];
+            finished.push_back(ran[i]);
+            if (hipMemcpyAsync((int32_t*)s->h_scores.p + (size_t)r.slot * nseq,
+                               (int32_t*)s->scores.p + (size_t)r.slot * nseq,
                                nseq * sizeof(int32_t), hipMemcpyDeviceToHost,
                                c->stream) != hipSuccess) {
                 s->loop_error = "score copy failed";
@@ -193,17 +253,23 @@ Round run_round(swh_server* s) {
         return Round::kFail;
     }
     const double round_ms = now_ms() - t_round;
+    if (round_ms > 0 && st.padded_cells > 0) {   // the rate the next budget is priced at
+        const double r = (double)st.padded_cells / round_ms;
+        s->rate = std::min(2.0e10, std::max(1.0e8, 0.7 * s->rate + 0.3 * r));
+    }
     std::lock_guard<std::mutex> lk(s->mu);
     s->stats.rounds += 1;
     s->stats.busy_ms += round_ms;
     s->stats.cells += st.cells;
-    for (size_t i = 0; i < batch.size(); ++i) {
-        Request& r = *batch[i];
+    for (size_t i = 0; i < ran.size(); ++i) {
+        Request& r = *ran[i];
+        const Task& T = tasks[i];
         r.next_slice += 1;
+        r.pair_pos = T.pair_end;
+        r.lane_pos = T.lane_end;
         r.info.rounds += 1;
-        r.info.cells += (uint64_t)r.query.size() *
-                        (range_residues(db, tasks[i].pair_begin, tasks[i].pair_end) +
-                         range_residues(db, tasks[i].lane_begin, tasks[i].lane_end));
+        r.info.cells += (uint64_t)r.query.size() * (range_residues(db, T.pair_begin, T.pair_end) +
+                                                    range_residues(db, T.lane_begin, T.lane_end));
     }
     for (auto& r : finished) {
         s->active.erase(std::find(s->active.begin(), s->active.end(), r));
@@ -285,11 +351,17 @@ int swh_server_create(swh_ctx* c, swh_db* db, const swh_server_opts* opts, swh_s
     if (opts) o = *opts;
     if (!o.max_inflight) o.max_inflight = 64;
     if (!o.memory_budget) o.memory_budget = uint64_t(4) << 30;
-    if (!o.slices) o.slices = 1;
     if (!o.max_qlen) o.max_qlen = 8192;
     if (!o.max_k) o.max_k = 100;
-    if (o.max_inflight < 1 || o.slices < 1 || o.max_qlen < 1 || o.max_k < 1)
+    if (o.round_ms == 0) o.round_ms = kDefaultRoundMs;
+    if (o.wait_weight == 0) o.wait_weight = kDefaultWaitWeight;
+    if (o.max_inflight < 1 || o.slices < 0 || o.max_qlen < 1 || o.max_k < 1 ||
+        !(o.round_ms > 0) || !(o.wait_weight >= 0))
         return fail(SWH_ERR_ARG, "bad server options");
+    // budgeted rounds on a database of one layout; a small database (two
+    // layouts, a whole round of a few milliseconds) runs whole rounds
+    const bool budgeted = o.slices == 0 && !db->alt;
+    if (o.slices == 0) o.slices = 1;
     int rc = check_opts(o.search);
     if (rc) return rc;
     HIP_TRY(hipSetDevice(c->device));
@@ -310,6 +382,7 @@ int swh_server_create(swh_ctx* c, swh_db* db, const swh_server_opts* opts, swh_s
     s->ctx = c;
     s->db = db;
     s->opts = o;
+    s->budgeted = budgeted;
     s->t0 = now_ms();
     s->pair_cut = cut_blocks(db, 0, db->n_pairs, o.slices);
     s->lane_cut = cut_blocks(db, db->n_pairs, db->n_blocks, o.slices);
@@ -318,6 +391,9 @@ int swh_server_create(swh_ctx* c, swh_db* db, const swh_server_opts* opts, swh_s
                          (size_t)(s->lane_cut[i + 1] - s->lane_cut[i]) * kLaneSeqs;
         s->max_slice_seqs = std::max(s->max_slice_seqs, n);
     }
+    if (budgeted)   // a slice may cover the whole database
+        s->max_slice_seqs = (size_t)db->n_pairs * 2 +
+                            (size_t)(db->n_blocks - db->n_pairs) * kLaneSeqs;
     if (o.slices == 1 && db->alt)   // a round may run on the second layout
         s->max_slice_seqs = std::max<size_t>(
             s->max_slice_seqs,
@@ -368,6 +444,8 @@ int swh_server_submit(swh_server* s, const uint8_t* query, int qlen, int k, int 
     r->k = k;
     r->want_scores = want_scores != 0;
     r->bytes = request_bytes(s, qlen);
+    r->pair_pos = 0;
+    r->lane_pos = s->db->n_pairs;
     if (r->bytes > s->opts.memory_budget)
         return fail(SWH_ERR_CAPACITY, "request needs %zu device bytes, above the server's "
                     "memory_budget %llu", r->bytes, (unsigned long long)s->opts.memory_budget);

@@ -179,3 +179,41 @@ def test_request_above_budget_refused_at_submit(setup):
     with Server(eng, d, memory_budget=db.n * 4 // 2) as srv:
         with pytest.raises(Exception, match="memory_budget"):
             srv.submit(queries(1, 97)[0], k=5)
+
+
+def test_budgeted_rounds_slice_and_stay_exact():
+    """Budgeted rounds (the default on a database of one layout): with 2 ms of
+    work per round, requests of 60-3000 residues submitted together are cut
+    into many slices of pair and lane blocks, shortest remaining work first.
+    Every score vector equals the request's search alone; the short requests
+    finish before the long ones; a sample equals the oracle."""
+    rng = np.random.Generator(np.random.PCG64(99))
+    lens = np.concatenate([G.lognormal_lengths(rng, 300_000, 330, 0.7, 20, 2500),
+                           G.loguniform_lengths(rng, 600, 2500, 12000)])
+    db = G.SeqDB.from_lengths(rng, lens)
+    eng = Engine("blosum62", 10, 2)
+    d = eng.load_db(db)
+    qs = G.queries([3000, 60, 900, 250, 1800, 120], 100)
+    alone = [eng.search(d, q)[0].copy() for q in qs]
+    with Server(eng, d, round_ms=2.0) as srv:
+        ids = [srv.submit(q, k=10, want_scores=True) for q in qs]
+        res = [srv.wait(i) for i in ids]
+        st = srv.stats()
+    d.close()
+    eng.close()
+    for i, (r, a) in enumerate(zip(res, alone)):
+        assert np.array_equal(r["scores"], a), f"request {i} (length {len(qs[i])})"
+        s, ix = topk(a, 10)
+        assert np.array_equal(r["top_score"], s) and np.array_equal(r["top_index"], ix)
+    # the longest request runs in several slices (each at least the round's
+    # two lane items per workgroup slot: 4096 lane blocks shared by the
+    # requests of the round, of this database's 2,344)
+    rounds = [r["info"]["rounds"] for r in res]
+    assert rounds[0] >= 3 and st["rounds"] >= max(rounds), rounds
+    done = [r["info"]["done_ms"] for r in res]
+    assert done[1] < done[0] and done[5] < done[0]      # 60 and 120 before 3000
+    idx = np.unique(np.concatenate([np.arange(db.n - 50, db.n),
+                                    rng.choice(db.n, 300, replace=False)]))
+    for q, r in zip(qs[:3], res[:3]):
+        want = O.search_i32(q, db.subset(idx), eng.matrix, 10, 2)
+        assert np.array_equal(r["scores"][idx], want)
