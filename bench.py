@@ -358,6 +358,32 @@ def run_dbfile(eng):
             "top10_equals_resident_search": same, "titles_ok": titles_ok}
 
 
+def pipeline_breakdown(db, qlen, chunk_bytes=0, runs=2):
+    """The worker pipeline (DataPool -> HIPWorker, no CPU worker) over the
+    5M-sequence database in host memory for one query length: where the
+    HIPWorker's time goes (first fill, aux fills and their swh_db_create
+    part, searches and their kernels, waits for the aux thread), from the
+    last of `runs` calls."""
+    from swhybrid_amd import datagen as G
+    from swhybrid_amd.engine import hybrid_search
+    q = G.queries([qlen], 5000 + qlen)[0]
+    for _ in range(runs):
+        r = hybrid_search(db, q, k=TOP_K, hip_workers=1, cpu_threads=0, chunk_bytes=chunk_bytes)
+    rep = r["report"]
+    cells = db.total * qlen
+    out = {"qlen": qlen, "chunk_bytes": chunk_bytes or 512 << 20,
+           "gcups_reference_measure": round(rep["gcups"], 1),
+           "gcups_wall": round(cells / rep["wall_seconds"] / 1e9, 1),
+           "seconds": round(rep["seconds"], 4), "wall_seconds": round(rep["wall_seconds"], 4)}
+    for k in ("dev_first_fill_s", "dev_aux_fill_s", "dev_pack_s", "dev_search_s",
+              "dev_aux_wait_s", "dev_kernel_s"):
+        out[k] = round(rep[k], 4)
+    out["dev_chunks"] = rep["dev_chunks"]
+    out["kernel_gcups"] = round(cells / rep["dev_kernel_s"] / 1e9, 1) if rep["dev_kernel_s"] else None
+    log(f"pipeline qlen {qlen}: {json.dumps(out)}")
+    return out
+
+
 def run_hybrid(db, qlens=(1000, 5478)):
     """The reference's hybrid scheduling: the worker pipeline over the same
     5M-sequence database held in host memory, chunks of up to 512 MB to the
@@ -491,6 +517,10 @@ def main():
                     help="priority credit of waiting (0 = the library default)")
     ap.add_argument("--serve-slices", type=int, default=0,
                     help="0 = budgeted rounds, n = every request in n fixed slices")
+    ap.add_argument("--pipeline", type=int, default=0, metavar="QLEN",
+                    help="only run the worker pipeline at this query length and print where "
+                         "its time goes")
+    ap.add_argument("--pipeline-chunk-mb", type=int, default=0)
     ap.add_argument("--cpu-sweep", action="store_true",
                     help="only measure the host scan's query segmentation and print it")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -519,6 +549,9 @@ def main():
 
     t0 = time.perf_counter()
     db = G.c2_db(n=args.nseq)
+    if args.pipeline:
+        print(json.dumps(pipeline_breakdown(db, args.pipeline, args.pipeline_chunk_mb << 20)))
+        return
     queries = G.c2_queries()
     if args.query_lengths:
         keep = {int(x) for x in args.query_lengths.split(",")}

@@ -78,6 +78,14 @@ typedef struct swhy_pool_report {
     double wall_seconds;       /* the whole call: worker set-up, the first chunk's
                                   fill and pack (outside `seconds`, as in the
                                   reference's timing), search, merge */
+    /* the HIPWorkers' time, summed over them: */
+    double dev_first_fill_s;   /* first DataPool fill + pack, before their loop */
+    double dev_aux_fill_s;     /* later fills + packs, on their aux threads */
+    double dev_pack_s;         /* of the fills, swh_db_create (upload, device packing) */
+    double dev_search_s;       /* swh_search calls */
+    double dev_aux_wait_s;     /* waits for the aux thread after a search */
+    double dev_kernel_s;       /* scan + rescore kernels of the searches */
+    int dev_chunks;            /* chunks searched */
 } swhy_pool_report;
 
 /* swhy_pool_search with a CPU worker beside the HIPWorkers.  out_top_* get

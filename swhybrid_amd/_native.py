@@ -188,7 +188,10 @@ class SwhyPoolReport(C.Structure):
     _fields_ = [("seconds", C.c_double), ("gcups", C.c_double), ("hip_residues", C.c_uint64),
                 ("cpu_residues", C.c_uint64), ("hip_busy_s", C.c_double),
                 ("cpu_busy_s", C.c_double), ("recalc", C.c_uint64),
-                ("wall_seconds", C.c_double)]
+                ("wall_seconds", C.c_double), ("dev_first_fill_s", C.c_double),
+                ("dev_aux_fill_s", C.c_double), ("dev_pack_s", C.c_double),
+                ("dev_search_s", C.c_double), ("dev_aux_wait_s", C.c_double),
+                ("dev_kernel_s", C.c_double), ("dev_chunks", C.c_int)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

@@ -1,5 +1,6 @@
 #include "HIPWorker.h"
 
+#include <chrono>
 #include <cstdio>
 #include <string>
 
@@ -17,7 +18,8 @@ HIPWorker::HIPWorker(DataPool* loader, Param* params, RecalcWorker* recalcWorker
       deviceId_(deviceId),
       ctx_(NULL),
       kernelMs_(0),
-      cells_(0) {
+      cells_(0),
+      packSeconds_(0) {
     dbs_[0] = dbs_[1] = NULL;
     bufferSize = chunkBytes < (size_t)maxReaderBufferSize ? chunkBytes
                                                           : (size_t)maxReaderBufferSize;
@@ -71,9 +73,12 @@ void HIPWorker::packDB() {
     dbs_[slot] = NULL;
     swh_db_opts dopt = {};
     dopt.queries_per_launch = 1;   // one query per search: one layout per chunk
+    const std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
     if (swh_db_create(ctx_, reinterpret_cast<const uint8_t*>(dbSeqBuf), offsets_.data(),
                       lengths_.data(), lengths_.size(), &dopt, &dbs_[slot]) != SWH_OK)
         throw swh_fail("swh_db_create");
+    packSeconds_ +=
+        std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     scores_[slot].resize(lengths_.size());
     e->setEntry(dbSeqBuf, NULL, scores_[slot].data());
     e->setBaseIdx(fillBaseIndex);

@@ -45,6 +45,8 @@ class HIPWorker : public Worker {
     // device time (scan + rescore kernels) and cells of this worker's searches
     double kernelSeconds() const { return kernelMs_ / 1e3; }
     uint64_t cells() const { return cells_; }
+    // wall time of the swh_db_create calls (chunk upload and device packing)
+    double packSeconds() const { return packSeconds_; }
 
   protected:
     void alloc() override;
@@ -65,6 +67,7 @@ class HIPWorker : public Worker {
     std::vector<uint32_t> lengths_;
     double kernelMs_;
     uint64_t cells_;
+    double packSeconds_;
 };
 
 #endif  // SWHYBRID_HIPWORKER_H_

@@ -61,7 +61,9 @@ void Worker::packDB() {
 
 void Worker::launch() {
     alloc();
+    double t = now();
     loaderFlag = fill();
+    phases_.first_fill += now() - t;
     if (!loaderFlag) {
         free();
         return;
@@ -71,6 +73,7 @@ void Worker::launch() {
     startTimer();
     while (loaderFlag) {
         pthread_create(&auxThread_, nullptr, auxEntryFunc, this);
+        t = now();
         try {
             search();
         } catch (...) {
@@ -78,7 +81,11 @@ void Worker::launch() {
             free();
             throw;
         }
+        const double t_searched = now();
         pthread_join(auxThread_, nullptr);
+        phases_.search += t_searched - t;
+        phases_.aux_wait += now() - t_searched;
+        phases_.chunks += 1;
         if (!error_.empty()) {
             free();
             throw error_.c_str();
@@ -98,7 +105,9 @@ void Worker::auxWork() {
         resultList.push(e->getScores(), e->getResultSize(), e->getBaseIdx());
         e->setResultSize(0);
     }
+    const double t = now();
     loaderFlag = fill();
+    phases_.aux_fill += now() - t;
 }
 
 void* Worker::auxEntryFunc(void* self) {

@@ -74,6 +74,14 @@ class Worker {
     void setScoreSink(int* sink) { sink_ = sink; }
     // seconds spent computing (device kernels or host scan)
     virtual double busySeconds() const { return 0; }
+    // Where the worker's time went: DataPool fills (packDB included) on the
+    // worker thread before the loop and on the aux thread, search() calls,
+    // waits for the aux thread after a search, and the chunks searched.
+    struct Phases {
+        double first_fill = 0, aux_fill = 0, search = 0, aux_wait = 0;
+        int chunks = 0;
+    };
+    const Phases& phases() const { return phases_; }
 
     int startThread();
     void waitForThread();
@@ -144,5 +152,6 @@ class Worker {
     pthread_t thread_{};
     pthread_t auxThread_{};
     double startTime_ = 0, stopTime_ = 0;
+    Phases phases_;
     std::string error_;
 };

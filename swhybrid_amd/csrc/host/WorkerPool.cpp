@@ -82,6 +82,27 @@ double WorkerPool::busySeconds(bool cpu) const {
     return s;
 }
 
+Worker::Phases WorkerPool::devicePhases() const {
+    Worker::Phases p;
+    for (size_t i = 0; i < workerList_.size(); ++i)
+        if (!isCpu_[i]) {
+            const Worker::Phases& w = workerList_[i]->phases();
+            p.first_fill += w.first_fill;
+            p.aux_fill += w.aux_fill;
+            p.search += w.search;
+            p.aux_wait += w.aux_wait;
+            p.chunks += w.chunks;
+        }
+    return p;
+}
+
+double WorkerPool::devicePackSeconds() const {
+    double s = 0;
+    for (size_t i = 0; i < workerList_.size(); ++i)
+        if (!isCpu_[i]) s += static_cast<const HIPWorker*>(workerList_[i])->packSeconds();
+    return s;
+}
+
 double WorkerPool::kernelSeconds() const {
     double s = 0;
     for (size_t i = 0; i < workerList_.size(); ++i)

@@ -36,6 +36,9 @@ class WorkerPool {
     // per kind: padded residues searched and seconds spent computing
     uint64_t residues(bool cpu) const;
     double busySeconds(bool cpu) const;
+    // the device workers' phases summed, and their swh_db_create time
+    Worker::Phases devicePhases() const;
+    double devicePackSeconds() const;
     uint64_t recalcCount() const { return recalcWorker_->getNum(); }
     const std::string& error() const { return error_; }
 

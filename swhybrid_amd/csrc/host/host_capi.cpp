@@ -61,6 +61,14 @@ int run_pool(DataPool& pool, uint64_t nseq, const int32_t* matrix, int gap_open,
         r.hip_busy_s = wp.busySeconds(false);
         r.cpu_busy_s = wp.busySeconds(true);
         r.recalc = wp.recalcCount();
+        const Worker::Phases ph = wp.devicePhases();
+        r.dev_first_fill_s = ph.first_fill;
+        r.dev_aux_fill_s = ph.aux_fill;
+        r.dev_pack_s = wp.devicePackSeconds();
+        r.dev_search_s = ph.search;
+        r.dev_aux_wait_s = ph.aux_wait;
+        r.dev_kernel_s = wp.kernelSeconds();
+        r.dev_chunks = ph.chunks;
         top = wp.result().getResult();
     }
     r.wall_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
