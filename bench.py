@@ -371,7 +371,8 @@ def pipeline_breakdown(db, qlen, chunk_bytes=0, runs=2):
         r = hybrid_search(db, q, k=TOP_K, hip_workers=1, cpu_threads=0, chunk_bytes=chunk_bytes)
     rep = r["report"]
     cells = db.total * qlen
-    out = {"qlen": qlen, "chunk_bytes": chunk_bytes or 512 << 20,
+    default = (1 << 30) if qlen >= 2700 else (512 << 20)   # the pool's default chunk
+    out = {"qlen": qlen, "chunk_bytes": chunk_bytes or default,
            "gcups_reference_measure": round(rep["gcups"], 1),
            "gcups_wall": round(cells / rep["wall_seconds"] / 1e9, 1),
            "seconds": round(rep["seconds"], 4), "wall_seconds": round(rep["wall_seconds"], 4)}
@@ -386,7 +387,8 @@ def pipeline_breakdown(db, qlen, chunk_bytes=0, runs=2):
 
 def run_hybrid(db, qlens=(1000, 5478)):
     """The reference's hybrid scheduling: the worker pipeline over the same
-    5M-sequence database held in host memory, chunks of up to 512 MB to the
+    5M-sequence database held in host memory, chunks of the pool's default
+    size (512 MB below query length 2,700, 1 GB from there) to the
     HIPWorker and of 4 MB to the CPU worker (the host's AVX2 cores), all
     from one DataPool; HIPWorker alone first, then both.  `*_gcups` is the
     reference's own measure (first worker start to last stop, after the

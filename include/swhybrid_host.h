@@ -21,8 +21,9 @@ extern "C" {
 const char* swhy_last_error(void);
 
 /* Search one query through the worker pipeline.
- *   chunk_bytes    DataPool chunk budget per fill (0 = 512 MB, twice the
- *                  reference's CUDAWorker bufferSize, src/CUDAWorker.cu:44)
+ *   chunk_bytes    DataPool chunk budget per fill (0 = 512 MB for queries
+ *                  shorter than 2,700 residues, 1 GB from there; the
+ *                  reference's CUDAWorker uses 256 MB, src/CUDAWorker.cu:44)
  *   n_workers      HIPWorkers on devices 0..n_workers-1
  *   out_top_*      topk entries: score, caller index, pool index; ties are
  *                  ordered by pool index (the order sequences are handed out),
@@ -64,7 +65,8 @@ typedef struct swhy_pool_opts {
     int topk;
     int hip_workers;           /* HIPWorkers on devices 0..hip_workers-1 */
     int cpu_threads;           /* threads of the CPU worker; 0 = no CPU worker */
-    uint64_t chunk_bytes;      /* HIPWorker chunk (0 = 512 MB) */
+    uint64_t chunk_bytes;      /* HIPWorker chunk (0 = 512 MB below query length
+                                  2,700, 1 GB from there) */
     uint64_t cpu_chunk_bytes;  /* CPU worker chunk (0 = 4 MB); the CPU worker takes no
                                   chunk once only hip_workers x chunk_bytes are left */
 } swhy_pool_opts;

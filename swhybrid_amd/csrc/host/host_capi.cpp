@@ -21,15 +21,19 @@
 
 namespace {
 thread_local std::string g_err;
-// HIPWorker chunk: twice the reference's 256 MB (src/CUDAWorker.cu:44).  One
-// launch per chunk, and on a 256-CU device a launch's tail is a smaller share
-// of a larger chunk; the first chunk's fill is not overlapped with a search,
-// so beyond a point larger chunks lose (5M-sequence pipeline, whole-call
-// GCUPS at query length 1000 / 5478: 4,726 / 6,688 at 256 MB, 4,982 / 7,417
-// at 512 MB, 4,612 / 7,650 at 1 GB).  Starting with smaller chunks loses more:
-// the pool hands out the longest sequences first, and a small launch of them
-// is mostly tail (4,658 / 6,161 from 64 MB doubling to 512 MB).
-const size_t kDefaultChunk = size_t(512) << 20;
+// HIPWorker chunk by default: 512 MB (twice the reference's 256 MB,
+// src/CUDAWorker.cu:44) for queries shorter than 2,700 residues, 1 GB from
+// there.  A chunk is one launch, and larger launches lose less to their tails
+// and to the uploads of the next chunk (5M sequences in host memory, query
+// length 1000 / 5478: kernels at 8,253 / 8,464 GCUPS with 512 MB chunks and
+// 9,089 / 9,132 with 1 GB; first worker start to last stop 7,040 / 8,220
+// against 7,916 / 8,907 GCUPS).  But the first chunk's fill, upload and packing
+// overlap nothing (48 ms at 512 MB, 98 ms at 1 GB), so for the whole call
+// 512 MB wins at 1000 (5,712 against 5,314 GCUPS) and 1 GB at 5478 (8,050
+// against 7,796); the two cross near query length 2,700.
+size_t default_chunk(int qlen) {
+    return qlen >= 2700 ? size_t(1) << 30 : size_t(512) << 20;
+}
 }
 
 namespace {
@@ -47,7 +51,7 @@ int run_pool(DataPool& pool, uint64_t nseq, const int32_t* matrix, int gap_open,
     swhy_pool_report r{};
     const auto t0 = std::chrono::steady_clock::now();
     {
-        WorkerPool wp(&pool, &params, o.chunk_bytes ? o.chunk_bytes : kDefaultChunk,
+        WorkerPool wp(&pool, &params, o.chunk_bytes ? o.chunk_bytes : default_chunk((int)pool.getQueryLen()),
                       o.cpu_chunk_bytes);
         const int rc = wp.run(out_scores ? sink.data() : nullptr);
         if (rc != 0) {

@@ -126,6 +126,7 @@ int swh_ctx_destroy(swh_ctx* c) {
     c->na_h_desc.release();
     for (auto& e : c->ev)
         if (e) (void)hipEventDestroy(e);
+
     if (c->stream) (void)hipStreamDestroy(c->stream);
     if (c->upload) (void)hipStreamDestroy(c->upload);
     delete c;
@@ -185,6 +186,17 @@ static bool enough_lane_items(const swh_ctx* c, uint64_t lane_blocks, int nq) {
 }
 
 }  // extern "C"
+
+// Whether p is page-locked host memory (hipHostMalloc, swh_pinned_alloc) that
+// a device copy can write directly.
+static bool is_pinned_host(const void* p) {
+    hipPointerAttribute_t a{};
+    if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+        (void)hipGetLastError();   // pageable memory: not an error of this call
+        return false;
+    }
+    return a.type == hipMemoryTypeHost;
+}
 
 // One layout of the database: sequences of at least group_min residues in
 // pair blocks, the rest in lane blocks.
@@ -298,19 +310,19 @@ static int create_layout(swh_ctx* c, const uint8_t* residues, const uint64_t* of
         for (uint64_t i = 0; i < nseq; ++i) soff[i] = lengths[i] > 0 ? offsets[i] - lo : 0;
     }
     // device buffers come from the context's spares where one fits and the
-    // staging buffers are the context's own: no hipFree here (it would wait
-    // for every search running on the device)
+    // staging buffers are the context's own and only grow: no hipFree here
+    // (it would wait for every search running on the device)
     auto up = [&](DevBuf& d, const void* h, size_t bytes) -> hipError_t {
-        hipError_t e = c->cache.take(d, std::max<size_t>(bytes, 16));
-        if (e != hipSuccess || bytes == 0) return e;
+        hipError_t r = c->cache.take(d, std::max<size_t>(bytes, 16));
+        if (r != hipSuccess || bytes == 0) return r;
         return hipMemcpy(d.p, h, bytes, hipMemcpyHostToDevice);
     };
     std::lock_guard<std::mutex> stage_lock(c->stage_mu);
     DevBuf& raw = c->stage_raw;
     DevBuf& d_soff = c->stage_soff;
     auto stage_up = [&](DevBuf& d, const void* h, size_t bytes) -> hipError_t {
-        hipError_t e = d.reserve(std::max<size_t>(bytes, 16));
-        if (e != hipSuccess || bytes == 0) return e;
+        hipError_t r = d.reserve_grow(std::max<size_t>(bytes, 16));
+        if (r != hipSuccess || bytes == 0) return r;
         return hipMemcpy(d.p, h, bytes, hipMemcpyHostToDevice);
     };
     hipError_t e = c->cache.take(db->words, std::max<uint64_t>(woff, 1) * 4);
@@ -326,7 +338,9 @@ static int create_layout(swh_ctx* c, const uint8_t* residues, const uint64_t* of
                         (const uint32_t*)db->perm.p, (const uint32_t*)db->sorted_len.p,
                         (const uint8_t*)raw.p, (const uint64_t*)d_soff.p,
                         (uint32_t*)db->words.p, c->upload);
-    if (e == hipSuccess) e = hipStreamSynchronize(c->upload);
+    // on failure too: no copy may still read the staging when this returns
+    const hipError_t se = hipStreamSynchronize(c->upload);
+    if (e == hipSuccess) e = se;
     if (raw.bytes > kStageKeepBytes) {   // a one-off large load: not kept
         raw.release();
         d_soff.release();
@@ -869,17 +883,6 @@ int enqueue_tasks(swh_ctx* c, swh_db* db, const Task* t, int nt, const Shape& sh
     return SWH_OK;
 }
 
-// Whether p is page-locked host memory (hipHostMalloc, swh_pinned_alloc) that
-// a device copy can write directly.
-static bool is_pinned_host(const void* p) {
-    hipPointerAttribute_t a{};
-    if (hipPointerGetAttributes(&a, p) != hipSuccess) {
-        (void)hipGetLastError();   // pageable memory: not an error of this call
-        return false;
-    }
-    return a.type == hipMemoryTypeHost;
-}
-
 // Search every query over the whole database in one launch sequence.
 // scores_host gets nq rows of nseq: copied there straight from the device
 // when it is page-locked, else through the context's pinned staging (up to 64
@@ -916,15 +919,15 @@ int run_queries(swh_ctx* c, swh_db* db_in, const uint8_t* const* qs, const int* 
     }
     const size_t nseq = db->nseq;
     const size_t score_bytes = (size_t)nq * nseq * sizeof(int32_t);
-    HIP_TRY(c->h_prof.reserve(prof_elems * sizeof(int16_t)));
-    HIP_TRY(c->prof.reserve(prof_elems * sizeof(int16_t)));
-    HIP_TRY(c->scores.reserve(score_bytes));
-    HIP_TRY(c->ovf.reserve((size_t)nq * nseq * sizeof(uint32_t)));
+    HIP_TRY(c->h_prof.reserve_grow(prof_elems * sizeof(int16_t)));
+    HIP_TRY(c->prof.reserve_grow(prof_elems * sizeof(int16_t)));
+    HIP_TRY(c->scores.reserve_grow(score_bytes));
+    HIP_TRY(c->ovf.reserve_grow((size_t)nq * nseq * sizeof(uint32_t)));
     HIP_TRY(c->counters.reserve((16 + (size_t)nq) * sizeof(uint32_t)));
     HIP_TRY(c->h_ctr.reserve((16 + (size_t)nq) * sizeof(uint32_t)));
     const bool direct_out = is_pinned_host(scores_host);
     const bool pinned_out = !direct_out && score_bytes <= (size_t(64) << 20);
-    if (pinned_out) HIP_TRY(c->h_scores.reserve(score_bytes));
+    if (pinned_out) HIP_TRY(c->h_scores.reserve_grow(score_bytes));
     int16_t* hp = (int16_t*)c->h_prof.p;
     std::vector<Task> tasks(nq);
     for (int i = 0; i < nq; ++i) {

@@ -71,6 +71,12 @@ struct DevBuf {
     void* p = nullptr;
     size_t bytes = 0;
     bool fixed = false;
+    // Grows to want, or to 1.5x the old size when that is more (hipFree waits
+    // for the device to be idle).
+    hipError_t reserve_grow(size_t want) {
+        if (want <= bytes) return hipSuccess;
+        return reserve(std::max(want, bytes + bytes / 2));
+    }
     static DevBuf view(void* at, size_t n) {
         DevBuf b;
         b.p = at;
@@ -167,6 +173,13 @@ struct PinnedBuf {
     void* p = nullptr;
     size_t bytes = 0;
     bool fixed = false;   // allocated once at set-up: a larger request fails
+    // Grows to want, or to 1.5x the old size when that is more: freeing
+    // page-locked memory waits for the device to be idle, so buffers refilled
+    // beside a running search (the next database's tables) should rarely grow.
+    hipError_t reserve_grow(size_t want) {
+        if (want <= bytes) return hipSuccess;
+        return reserve(std::max(want, bytes + bytes / 2));
+    }
     hipError_t reserve(size_t want) {
         if (want <= bytes) return hipSuccess;
         if (fixed) return hipErrorOutOfMemory;
