@@ -188,6 +188,30 @@ def test_blosum_tables():
     assert A.BLOSUM50[A.CODE_ORDER.index("C"), A.CODE_ORDER.index("C")] == 13
 
 
+def test_blosum50_textbook_alignments():
+    """Known answers from Durbin, Eddy, Krogh & Mitchison, Biological Sequence
+    Analysis (1998), section 2.3: HEAGAWGHEE against PAWHEAE under BLOSUM50 and
+    a linear gap cost of 8.  The best global alignment (HEAGAWGHE-E /
+    --P-AW-HEAE) scores 1 and the best local one (AWGHE / AW-HE) 28.  A wrong
+    entry on either alignment changes its score, and an entry among H, E, A,
+    G, W, P that is too high can raise an optimum.  A gap of k residues costs
+    open + k x extend here, so linear 8 is 0/8."""
+    x, y = "HEAGAWGHEE", "PAWHEAE"
+    m = A.BLOSUM50
+    s = lambda a, b: int(m[A.CODE_ORDER.index(a), A.CODE_ORDER.index(b)])
+    # Needleman-Wunsch with a linear gap cost (the textbook's recurrence)
+    F = np.zeros((len(x) + 1, len(y) + 1), dtype=np.int64)
+    F[:, 0] = -8 * np.arange(len(x) + 1)
+    F[0, :] = -8 * np.arange(len(y) + 1)
+    for i in range(1, len(x) + 1):
+        for j in range(1, len(y) + 1):
+            F[i, j] = max(F[i - 1, j - 1] + s(x[i - 1], y[j - 1]), F[i - 1, j] - 8,
+                          F[i, j - 1] - 8)
+    assert F[-1, -1] == 1
+    assert O.gotoh(A.encode(x), A.encode(y), m, 0, 8) == 28
+    assert O.gotoh(A.encode(y), A.encode(x), m, 0, 8) == 28
+
+
 @pytest.mark.skipif(not os.path.exists(REF_PARAM_H), reason="reference checkout not present")
 def test_blosum62_equals_reference_table():
     text = open(REF_PARAM_H).read()
