@@ -421,7 +421,8 @@ def run_hybrid(db, qlens=(1000, 5478)):
     return out
 
 
-def run_serving(eng, d, qps_levels, n_requests, round_ms=0.0, slices=0, wait_weight=0.0):
+def run_serving(eng, d, qps_levels, n_requests, round_ms=0.0, slices=0, wait_weight=0.0,
+                queue="auto"):
     """Poisson arrivals at fixed QPS, lengths logN(mean 400, sigma 0.7) clipped
     to [30, 5000], top-20 per request, the database resident; p50/p99 latency
     (submit to top-k delivered) and sustained GCUPS per level.  Each request
@@ -442,7 +443,8 @@ def run_serving(eng, d, qps_levels, n_requests, round_ms=0.0, slices=0, wait_wei
                 t0 = time.perf_counter()
                 eng.search(d, q)
                 alone_ms[key] = (time.perf_counter() - t0) * 1e3
-        srv = Server(eng, d, round_ms=round_ms, slices=slices, wait_weight=wait_weight)
+        srv = Server(eng, d, round_ms=round_ms, slices=slices, wait_weight=wait_weight,
+                     queue=queue)
         ids = [None] * len(qs)
         t0 = time.perf_counter()
 
@@ -487,7 +489,7 @@ def run_serving(eng, d, qps_levels, n_requests, round_ms=0.0, slices=0, wait_wei
             "gcups": round(cells / span / 1e9, 1), "offered_tcups": round(
                 qps * np.mean([len(q) for q in qs]) * d.residues / 1e12, 2),
             "peak_inflight": st["peak_inflight"], "rounds": st["rounds"],
-            "max_rounds_per_request": int(max(rounds))}
+            "max_rounds_per_request": int(max(rounds)), "queue": queue}
         r_ = res[list(res)[-1]]
         log(f"serving {qps} QPS: p50 {r_['p50_ms']} ms, p99 {r_['p99_ms']} ms (alone "
             f"{r_['alone_p50_ms']} / {r_['alone_p99_ms']}), slowdown p50 {r_['slowdown_p50']} "
@@ -512,6 +514,8 @@ def main():
     ap.add_argument("--no-overflow-stress", action="store_true",
                     help="skip the 2M-sequence near-copy workload")
     ap.add_argument("--serve-qps", type=str, default="5,20,50")
+    ap.add_argument("--serve-queue", choices=["auto", "live", "rounds"], default="auto",
+                    help="serving loop: live queue (the default on this database) or rounds")
     ap.add_argument("--serve-requests", type=int, default=60)
     ap.add_argument("--serve-round-ms", type=float, default=0.0,
                     help="device work per serving round in ms (0 = the library default)")
@@ -660,7 +664,8 @@ def main():
     if not args.no_serve:
         out["served"] = run_serving(eng, d, [float(x) for x in args.serve_qps.split(",")],
                                     args.serve_requests, args.serve_round_ms,
-                                    args.serve_slices, args.serve_wait_weight)
+                                    args.serve_slices, args.serve_wait_weight,
+                                    args.serve_queue)
     d.close()
     eng.close()
     if not args.no_overflow_stress:

@@ -226,12 +226,15 @@ int swh_get_tuning(const swh_ctx* ctx, swh_tune_entry* entries, int cap, int* n)
 /* ---- Serving: continuous batching on a resident database ------------------
  * A server owns a loop thread.  Requests are admitted, in arrival order, when
  * the device memory their query profile, score vector and overflow list need
- * fits the budget; each round then runs ONE persistent scan launch over the
- * next database slice of every admitted request, so a request that arrives
- * while others are on the GPU joins them at the next round and their work
- * overlaps on the device.  Scores are exact and independent of which
- * requests share a round.  No reference counterpart (the reference runs one
- * query per process). */
+ * fits the budget.  With the live queue (the default on a database of one
+ * layout) ONE persistent scan launch stays on the GPU while there is work:
+ * a request admitted while others run has its work items appended to that
+ * launch's device queue at once, between theirs (shortest remaining work
+ * first), and its scores are copied out as soon as its own items are done.
+ * With rounds, each round runs one launch over the next database slice of
+ * every admitted request, and an arriving request joins at the next round.
+ * Scores are exact and independent of which requests share the device.  No
+ * reference counterpart (the reference runs one query per process). */
 typedef struct swh_server swh_server;
 
 typedef struct swh_server_opts {
@@ -250,23 +253,36 @@ typedef struct swh_server_opts {
     swh_search_opts search;    /* strip widths, overflow threshold */
     double round_ms;           /* budgeted rounds: device work per round in ms
                                   (0 = 160) */
-    double wait_weight;        /* budgeted rounds: a request's priority is its
-                                  remaining work less wait_weight x the work the
-                                  device could have done while it waited (0 =
-                                  0.5; larger favours older requests) */
+    double wait_weight;        /* live queue and budgeted rounds: a request's
+                                  priority is its remaining work less
+                                  wait_weight x the work the device could have
+                                  done while it waited (0 = 0.5; larger favours
+                                  older requests) */
+    int queue;                 /* SWH_QUEUE_AUTO (0): the live queue on a
+                                  database of one layout with slices = 0, rounds
+                                  otherwise; SWH_QUEUE_LIVE: one persistent scan
+                                  launch stays on the device while there is
+                                  work and requests join it as they arrive
+                                  (slices and round_ms unused; a small database
+                                  is searched on its first layout);
+                                  SWH_QUEUE_ROUNDS: one launch per round */
 } swh_server_opts;
+
+#define SWH_QUEUE_AUTO 0
+#define SWH_QUEUE_LIVE 1
+#define SWH_QUEUE_ROUNDS 2
 
 typedef struct swh_request_info {
     double submit_ms, admit_ms, done_ms;   /* server clock: ms since creation */
     double latency_ms;                     /* done - submit */
     uint64_t cells;
-    int rounds;
+    int rounds;                            /* rounds (live queue: launches) it took part in */
     int status;                            /* SWH_OK or the failure code */
 } swh_request_info;
 
 typedef struct swh_server_stats {
-    uint64_t rounds, completed, cells;
-    double busy_ms;                        /* wall time spent in rounds */
+    uint64_t rounds, completed, cells;     /* rounds: rounds run (live queue: launches begun) */
+    double busy_ms;                        /* wall time spent in rounds (launches) */
     int peak_inflight;
     int waiting;                           /* submitted, not yet admitted */
 } swh_server_stats;

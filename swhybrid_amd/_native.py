@@ -70,7 +70,11 @@ class SwhServerOpts(C.Structure):
         ("search", SwhSearchOpts),
         ("round_ms", C.c_double),
         ("wait_weight", C.c_double),
+        ("queue", C.c_int),
     ]
+
+
+QUEUE = {"auto": 0, "live": 1, "rounds": 2}
 
 
 class SwhRequestInfo(C.Structure):

@@ -478,7 +478,7 @@ static double group_lane_use(int qlen, int W, int max_passes, int min_lanes) {
 // workgroups of 4 waves per CU) can keep a border of its longest pair for
 // each of its 64 / G groups within the context's border budget, so that more
 // passes never shrink the grid.
-static int group_min_lanes(const swh_ctx* c, uint32_t rows_max, uint64_t border_budget) {
+int group_min_lanes(const swh_ctx* c, uint32_t rows_max, uint64_t border_budget) {
     const uint64_t per_group = (uint64_t)std::max(c->num_cus, 1) * 3 * kWavesPerWG *
                                sizeof(uint2) * std::max<uint32_t>(rows_max, 1);
     const uint64_t m = std::max<uint64_t>(1, std::min<uint64_t>(kWave, border_budget / per_group));
@@ -502,7 +502,7 @@ struct SmallBuf {
     const T& operator[](size_t i) const { return p[i]; }
 };
 
-static int group_pass_budget(const swh_ctx* c, uint64_t launch_cells, uint32_t rows_max, int W) {
+int group_pass_budget(const swh_ctx* c, uint64_t launch_cells, uint32_t rows_max, int W) {
     if (rows_max == 0) return kWave;
     const double per_lane = (double)launch_cells / (2.0 * 512.0 * std::max(c->num_cus, 1));
     const double item = ((double)rows_max + kWave) * W;
@@ -559,8 +559,8 @@ static int tune_bucket(const swh_ctx* c, int qlen) {
 // bucket when that was measured at this group width (measured at the bucket's
 // longest query, so at most one lane per strip of a shorter one), else the
 // best lane use within the launch's pass budget (group_lanes_for).
-static int group_lanes(const swh_ctx* c, int qlen, int Wg, const swh_search_opts& o,
-                       int max_passes, int min_lanes) {
+int group_lanes(const swh_ctx* c, int qlen, int Wg, const swh_search_opts& o, int max_passes,
+                int min_lanes) {
     if (o.group_lanes > 0) return std::min(o.group_lanes, kWave);
     const int b = tune_bucket(c, qlen);
     if (b >= 0 && c->tune[b].group_width == Wg && c->tune[b].group_lanes > 0)
@@ -658,6 +658,57 @@ Shape choose_shape(const swh_ctx* c, const int* qlens, int nq, const swh_search_
     return sh;
 }
 
+// Work items longest first across every task (LPT order): a merge of the
+// per-task lists, each already longest first because blocks are sorted by
+// length.  Duration model in VALU ops per lane: a lane item runs its first
+// (longest) block's rows through every strip at 9.5 ops per packed cell plus
+// 13 per strip row; a group item runs its first pair's rows (plus the lane
+// skew of group_step_rows rows per lane) through every pass at 9.5 ops per
+// cell plus 17 per row.
+void order_items(const swh_db* db, const Shape& sh, const QueryDesc* lane_d,
+                 const QueryDesc* group_d, int nt, uint2* out) {
+    auto item_cost = [&](int list, int k) -> double {
+        if (list < nt) {
+            const QueryDesc& Gd = group_d[list];
+            const int p = Gd.blk_begin + k * kWavesPerWG * (kWave / Gd.group);
+            const int npass = (Gd.qpad / sh.Wg + Gd.group - 1) / Gd.group;
+            const int nr = group_step_rows(sh.Wl, sh.Wg);
+            return (double)npass * (db->blocks[p].rows + nr * (Gd.group - 1)) *
+                   (9.5 * sh.Wg + 17.0);
+        }
+        const QueryDesc& L = lane_d[list - nt];
+        const int b = L.blk_begin + k * kWavesPerWG;
+        const int strips = (L.qpad + sh.Wl - 1) / sh.Wl;
+        return (double)db->blocks[b].rows * (9.5 * L.qpad + 13.0 * strips);
+    };
+    struct Head {
+        double cost;
+        int list, k;
+        bool operator<(const Head& h) const {   // max-heap on cost, then list order
+            return cost != h.cost ? cost < h.cost : list > h.list;
+        }
+    };
+    SmallBuf<Head, 16> heap(2 * (size_t)nt);
+    size_t nh = 0;
+    for (int l = 0; l < 2 * nt; ++l)
+        if ((l < nt ? group_d[l].n_items : lane_d[l - nt].n_items) > 0)
+            heap[nh++] = Head{item_cost(l, 0), l, 0};
+    std::make_heap(heap.p, heap.p + nh);
+    for (int n = 0; nh > 0; ++n) {
+        std::pop_heap(heap.p, heap.p + nh);
+        Head& h = heap[nh - 1];
+        out[n] = h.list < nt ? make_uint2((uint32_t)h.list, (uint32_t)h.k)
+                             : make_uint2((uint32_t)(h.list - nt) | kLaneItem, (uint32_t)h.k);
+        const int n_list = h.list < nt ? group_d[h.list].n_items : lane_d[h.list - nt].n_items;
+        if (++h.k < n_list) {
+            h.cost = item_cost(h.list, h.k);
+            std::push_heap(heap.p, heap.p + nh);
+        } else {
+            --nh;
+        }
+    }
+}
+
 // Enqueue, without waiting: counter reset, one scan launch for every task and
 // the int32 rescore of their overflow lists.  counters needs 16 + nt words;
 // counters[16 + i] ends up holding task i's overflow count.
@@ -741,41 +792,7 @@ int enqueue_tasks(swh_ctx* c, swh_db* db, const Task* t, int nt, const Shape& sh
             range_padded(db, T.lane_begin, T.lane_end) * (uint64_t)L.qpad +
             range_padded(db, T.pair_begin, T.pair_end) * (uint64_t)passes * Gd.group * sh.Wg;
     }
-    // Work items longest first across every task (LPT order): a merge of the
-    // per-task lists, each already longest first because blocks are sorted by
-    // length.  Duration model in VALU ops per lane: a lane item runs its first
-    // (longest) block's rows through every strip at 9.5 ops per packed cell
-    // plus 13 per strip row; a group item runs its first pair's rows (plus the
-    // lane skew of group_step_rows rows per lane) through every pass at 9.5
-    // ops per cell plus 17 per row.
     const int n_items = items_g + items_l;
-    auto item_cost = [&](int list, int k) -> double {
-        if (list < nt) {
-            const QueryDesc& Gd = hd[nt + list];
-            const int p = Gd.blk_begin + k * kWavesPerWG * (kWave / Gd.group);
-            const int npass = (Gd.qpad / sh.Wg + Gd.group - 1) / Gd.group;
-            const int nr = group_step_rows(sh.Wl, sh.Wg);
-            return (double)npass * (db->blocks[p].rows + nr * (Gd.group - 1)) *
-                   (9.5 * sh.Wg + 17.0);
-        }
-        const QueryDesc& L = hd[list - nt];
-        const int b = L.blk_begin + k * kWavesPerWG;
-        const int strips = (L.qpad + sh.Wl - 1) / sh.Wl;
-        return (double)db->blocks[b].rows * (9.5 * L.qpad + 13.0 * strips);
-    };
-    struct Head {
-        double cost;
-        int list, k;
-        bool operator<(const Head& h) const {   // max-heap on cost, then list order
-            return cost != h.cost ? cost < h.cost : list > h.list;
-        }
-    };
-    SmallBuf<Head, 16> heap(2 * (size_t)nt);
-    size_t nh = 0;
-    for (int l = 0; l < 2 * nt; ++l)
-        if ((l < nt ? hd[nt + l].n_items : hd[l - nt].n_items) > 0)
-            heap[nh++] = Head{item_cost(l, 0), l, 0};
-    std::make_heap(heap.p, heap.p + nh);
     const size_t desc_bytes = 3 * (size_t)nt * sizeof(QueryDesc);
     const size_t total_bytes = desc_bytes + (size_t)n_items * sizeof(uint2);
     if ((lr.h_desc->fixed && total_bytes > lr.h_desc->bytes) ||
@@ -785,20 +802,7 @@ int enqueue_tasks(swh_ctx* c, swh_db* db, const Task* t, int nt, const Shape& sh
     HIP_TRY(lr.h_desc->reserve(total_bytes));
     HIP_TRY(lr.desc->reserve(total_bytes));
     std::memcpy(lr.h_desc->p, hd.p, desc_bytes);
-    uint2* hitems = (uint2*)((char*)lr.h_desc->p + desc_bytes);
-    for (int n = 0; nh > 0; ++n) {
-        std::pop_heap(heap.p, heap.p + nh);
-        Head& h = heap[nh - 1];
-        hitems[n] = h.list < nt ? make_uint2((uint32_t)h.list, (uint32_t)h.k)
-                                : make_uint2((uint32_t)(h.list - nt) | kLaneItem, (uint32_t)h.k);
-        const int n_list = h.list < nt ? hd[nt + h.list].n_items : hd[h.list - nt].n_items;
-        if (++h.k < n_list) {
-            h.cost = item_cost(h.list, h.k);
-            std::push_heap(heap.p, heap.p + nh);
-        } else {
-            --nh;
-        }
-    }
+    order_items(db, sh, &hd[0], &hd[nt], nt, (uint2*)((char*)lr.h_desc->p + desc_bytes));
     *lr.desc_used = total_bytes;
     HIP_TRY(hipMemcpyAsync(lr.desc->p, lr.h_desc->p, total_bytes, hipMemcpyHostToDevice,
                            lr.stream));
@@ -822,20 +826,10 @@ int enqueue_tasks(swh_ctx* c, swh_db* db, const Task* t, int nt, const Shape& sh
     if (per_wg)
         grid = (int)std::min<uint64_t>(grid, std::max<uint64_t>(1, lr.border_budget / per_wg));
     grid = std::max(1, std::min(grid, items));
-    // rescore: a lane group per sequence, one pass's profile columns in LDS;
-    // only queries swept in more than one pass keep a border per group, and
-    // the grid is capped so those borders fit 1 GiB
-    const int ps_r = max_pass_r + kProfPad;
-    const size_t lds_r = (size_t)kProfRows * ps_r * sizeof(int16_t);
-    const uint32_t lmax = std::max(db->max_rows_lane, db->max_rows_pair);
-    const uint64_t rstride = (uint64_t)max_gpw_r * lmax;
-    const uint64_t r_per_wg = rstride * sizeof(uint2) * kWavesPerWG;
-    int rgrid = c->num_cus * rescore_max_wg_per_cu(lds_r);
-    const uint64_t rbudget = std::min<uint64_t>(size_t(1) << 30, lr.border_budget);
-    if (r_per_wg)
-        rgrid = (int)std::max<uint64_t>(1, std::min<uint64_t>(rgrid, rbudget / r_per_wg));
+    RescoreLaunch rl = plan_rescore(c, db, max_pass_r, max_gpw_r, lr.border_budget, d_res, nt,
+                                    counters + 2);
     const uint64_t border_need =
-        std::max<uint64_t>({16, per_wg * (uint64_t)grid, r_per_wg * (uint64_t)rgrid});
+        std::max<uint64_t>({16, per_wg * (uint64_t)grid, rl.per_wg * (uint64_t)rl.grid});
     if (lr.border->fixed && border_need > lr.border->bytes)
         return fail(SWH_ERR_CAPACITY, "strip borders need %llu bytes for one workgroup, above "
                     "the fixed %zu", (unsigned long long)border_need, lr.border->bytes);
@@ -847,6 +841,7 @@ int enqueue_tasks(swh_ctx* c, swh_db* db, const Task* t, int nt, const Shape& sh
     a.perm = (const uint32_t*)db->perm.p;
     a.group_q = d_group;
     a.lane_q = d_lane;
+    a.desc_stride = 1;
     a.items = d_items;
     a.work_counter = counters + 0;
     a.border = (uint2*)lr.border->p;
@@ -862,25 +857,47 @@ int enqueue_tasks(swh_ctx* c, swh_db* db, const Task* t, int nt, const Shape& sh
         HIP_TRY(launch_scan(sh.Wl, sh.Wg, sh.pair, a, grid, lds, lr.stream));
         st->n_launches += 1;
     }
-    RescoreArgs r{};
-    r.words = a.words;
-    r.blocks = a.blocks;
-    r.perm = a.perm;
-    r.sorted_len = (const uint32_t*)db->sorted_len.p;
-    r.sorted_blk = (const uint32_t*)db->sorted_blk.p;
-    r.queries = d_res;
-    r.work_counter = counters + 2;
-    r.border = (uint2*)lr.border->p;
-    r.border_stride = rstride;
-    r.border_rows = lmax;
-    r.n_queries = nt;
-    r.n_lane_blocks_first = db->n_pairs;
-    r.ps = ps_r;
-    r.gap_oe = gap_oe;
-    r.gap_e = c->gap_extend;
-    HIP_TRY(launch_rescore(r, rgrid, lds_r, lr.stream));
+    rl.args.border = (uint2*)lr.border->p;
+    HIP_TRY(launch_rescore(rl.args, rl.grid, rl.lds, lr.stream));
     st->n_launches += 1;
     return SWH_OK;
+}
+
+// The int32 rescore of nq queries' overflow lists: a lane group per
+// sequence, one pass's profile columns in LDS; only queries swept in more
+// than one pass keep a border per group, and the grid is capped so those
+// borders fit 1 GiB.  max_pass_r: the widest pass (group x 8 columns);
+// max_gpw_r: lane groups per wave of the multi-pass queries (0: none).  The
+// caller sets args.border once the border buffer holds per_wg x grid bytes.
+RescoreLaunch plan_rescore(const swh_ctx* c, const swh_db* db, int max_pass_r, int max_gpw_r,
+                           uint64_t border_budget, const QueryDesc* d_queries, int nq,
+                           uint32_t* work_counter) {
+    RescoreLaunch rl{};
+    const int ps_r = max_pass_r + kProfPad;
+    rl.lds = (size_t)kProfRows * ps_r * sizeof(int16_t);
+    const uint32_t lmax = std::max(db->max_rows_lane, db->max_rows_pair);
+    const uint64_t rstride = (uint64_t)max_gpw_r * lmax;
+    rl.per_wg = rstride * sizeof(uint2) * kWavesPerWG;
+    rl.grid = c->num_cus * rescore_max_wg_per_cu(rl.lds);
+    const uint64_t rbudget = std::min<uint64_t>(size_t(1) << 30, border_budget);
+    if (rl.per_wg)
+        rl.grid = (int)std::max<uint64_t>(1, std::min<uint64_t>(rl.grid, rbudget / rl.per_wg));
+    RescoreArgs& r = rl.args;
+    r.words = (const uint32_t*)db->words.p;
+    r.blocks = (const BlockDesc*)db->d_blocks.p;
+    r.perm = (const uint32_t*)db->perm.p;
+    r.sorted_len = (const uint32_t*)db->sorted_len.p;
+    r.sorted_blk = (const uint32_t*)db->sorted_blk.p;
+    r.queries = d_queries;
+    r.work_counter = work_counter;
+    r.border_stride = rstride;
+    r.border_rows = lmax;
+    r.n_queries = nq;
+    r.n_lane_blocks_first = db->n_pairs;
+    r.ps = ps_r;
+    r.gap_oe = c->gap_open + c->gap_extend;
+    r.gap_e = c->gap_extend;
+    return rl;
 }
 
 // Search every query over the whole database in one launch sequence.

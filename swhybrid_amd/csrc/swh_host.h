@@ -348,6 +348,26 @@ int refuse_if_served(const swh_ctx* c, const char* what);
 int enqueue_tasks(swh_ctx* c, swh_db* db, const Task* t, int nt, const Shape& sh,
                   const swh_search_opts& o, uint32_t* counters, const LaunchRes& lr,
                   swh_stats* st);
+// Work items of nt tasks (lane descriptors lane_d[i], group descriptors
+// group_d[i], n_items set) in launch order, longest first: x = task (|
+// kLaneItem for a lane item), y = item of the task.
+void order_items(const swh_db* db, const Shape& sh, const QueryDesc* lane_d,
+                 const QueryDesc* group_d, int nt, uint2* out);
+// Group-mode shape helpers of the launch planning (swh_capi.cpp).
+int group_min_lanes(const swh_ctx* c, uint32_t rows_max, uint64_t border_budget);
+int group_pass_budget(const swh_ctx* c, uint64_t launch_cells, uint32_t rows_max, int W);
+int group_lanes(const swh_ctx* c, int qlen, int Wg, const swh_search_opts& o, int max_passes,
+                int min_lanes);
+// An int32 rescore launch and the strip-border bytes it needs.
+struct RescoreLaunch {
+    RescoreArgs args;
+    int grid;
+    size_t lds;
+    uint64_t per_wg;   // border bytes per workgroup
+};
+RescoreLaunch plan_rescore(const swh_ctx* c, const swh_db* db, int max_pass_r, int max_gpw_r,
+                           uint64_t border_budget, const QueryDesc* d_queries, int nq,
+                           uint32_t* work_counter);
 // Search every query over the whole database; scores_host gets nq rows of nseq.
 int run_queries(swh_ctx* c, swh_db* db, const uint8_t* const* qs, const int* qlens, int nq,
                 const swh_search_opts& o, int32_t* scores_host, swh_stats* stats);

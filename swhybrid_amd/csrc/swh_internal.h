@@ -54,12 +54,46 @@ struct QueryDesc {
                              // table, lane blocks in the lane table)
 };
 
+// Live queue of the serving scan (k_scan with LIVE = true): the host appends
+// work items while the kernel runs.  A workgroup draws ticket t from the work
+// counter; its item is ring[t & (cap - 1)] once the low half of ctl (items
+// published so far) exceeds t.  Until then it polls ctl, one 64-bit read with
+// sleeps growing to about 100 us between reads (512 workgroups polling page-
+// locked memory every few microseconds took the PCIe bandwidth the working
+// ones needed), and it leaves without an item once the high half (closed) is
+// set and t is still unpublished, or after idle_ticks of the 100 MHz
+// real-time counter, when it also sets status[0].
+// After an item the workgroup writes its results back to memory (a
+// system-scope release) and stores ((t + 1) << 1 | overflowed) into
+// done[t & (cap - 1)], so the host sees a task's scores while the kernel runs
+// on.  ctl, ring, done and status are host-coherent page-locked memory,
+// accessed with system-scope atomics.  A task's descriptor block and profile
+// are written by the host into a page-locked arena; the task's first item (a
+// prep item) copies them to the same offsets of a device arena, and the host
+// publishes the task's scan items once that item is done, so scans read them
+// from HBM.  Each launch fills both arenas from the start: nothing the kernel
+// reads there was read earlier in the launch (no stale cache line holds it).
+struct LiveQueue {
+    const uint64_t* ctl;       // published | closed << 32
+    const uint2* ring;
+    uint32_t* done;
+    uint32_t* status;
+    uint32_t cap;              // a power of two
+    uint64_t idle_ticks;
+    const char* host_arena;    // device address of the page-locked arena
+    char* dev_arena;           // the device arena (same layout)
+};
+// QueryDescs per task in a live launch: lane, group and rescore descriptor
+// and one unused, 256 bytes, so no two tasks share a cache line.
+constexpr int kLiveDescStride = 4;
+
 struct ScanArgs {
     const uint32_t* words;     // packed residues
     const BlockDesc* blocks;   // block table: pair blocks, then lane blocks
     const uint32_t* perm;      // sorted position -> caller index (kNoSeq for fillers)
     const QueryDesc* group_q;  // per query: group-item range, G, qpad for the group width
     const QueryDesc* lane_q;   // per query: lane-item range, qpad for the lane width
+    int desc_stride;           // QueryDescs from one query's descriptor to the next's
     const uint2* items;        // work queue order: x = query | kLaneItem, y = item of the query
     uint32_t* work_counter;
     uint2* border;             // per wave slot, border_stride uint2
@@ -69,6 +103,7 @@ struct ScanArgs {
     int n_items;
     int ps_lane, ps_group;     // LDS profile row strides (int16)
     int gap_oe, gap_e;
+    LiveQueue live;            // k_scan<..., LIVE = true> only
 };
 
 // Work items of the search kernel: group items (G lanes cooperate on one long
@@ -79,6 +114,7 @@ struct ScanArgs {
 // one query, one per wave), handed out longest first across every query of
 // the launch.  Rescore items are 256 overflow-list entries.
 constexpr uint32_t kLaneItem = 0x80000000u;
+constexpr uint32_t kPrepItem = 0x40000000u;   // live queue: stage a task's descriptors and profile
 
 struct RescoreArgs {
     const uint32_t* words;
@@ -123,6 +159,10 @@ hipError_t launch_pack(const BlockDesc* blocks, int n_blocks, int n_pairs, const
 hipError_t launch_scan(int lane_width, int group_width, bool pair, const ScanArgs& a, int grid,
                        size_t lds_bytes, hipStream_t s);
 int scan_max_wg_per_cu(int lane_width, int group_width, bool pair, size_t lds_bytes);
+// The same scan fed through a.live (swh_kernels_live.hip).
+hipError_t launch_scan_live(int lane_width, int group_width, bool pair, const ScanArgs& a,
+                            int grid, size_t lds_bytes, hipStream_t s);
+int scan_live_max_wg_per_cu(int lane_width, int group_width, bool pair, size_t lds_bytes);
 hipError_t launch_rescore(const RescoreArgs& a, int grid, size_t lds_bytes, hipStream_t s);
 int rescore_max_wg_per_cu(size_t lds_bytes);
 // LDS the lane path of the search kernel needs for a strip width.

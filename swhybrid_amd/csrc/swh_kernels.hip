@@ -334,14 +334,17 @@ __device__ __forceinline__ void lane_strip_any(const uint2* wds, int quads, uint
     }
 }
 
+// Score of sorted sequence s, or its overflow-list entry (and *ovf = 1, an
+// LDS flag of the workgroup's item).
 __device__ __forceinline__ void emit(const QueryDesc& Qd, const uint32_t* perm, uint32_t s,
-                                     int sc) {
+                                     int sc, int* ovf) {
     const uint32_t idx = perm[s];
     if (idx == kNoSeq) return;
     if (sc >= Qd.ovf_threshold) {
         const uint32_t slot = atomicAdd(Qd.ovf_count, 1u);
         Qd.ovf_list[slot] = s;
         Qd.scores[idx] = -1;
+        *ovf = 1;
     } else {
         Qd.scores[idx] = sc;
     }
@@ -381,7 +384,7 @@ __device__ __forceinline__ void with_strip_width(int wl, F&& f) {
 template <int W, bool PAIR>
 __device__ __forceinline__ void lane_item(const ScanArgs& A, const QueryDesc& Qd, int q,
                                           int local, short* lds, uint2* bord, int& loaded_q,
-                                          int& loaded_seg) {
+                                          int& loaded_seg, int* ovf) {
     const int wave = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
     const s2 Q = {(short)A.gap_oe, (short)A.gap_oe};
@@ -436,8 +439,8 @@ __device__ __forceinline__ void lane_item(const ScanArgs& A, const QueryDesc& Qd
         loaded_q = -1;   // the single-profile staging area was overwritten
     }
     if (active) {
-        emit(Qd, A.perm, bd.first + lane, (int)S.x + 32768);
-        emit(Qd, A.perm, bd.first + kWave + lane, (int)S.y + 32768);
+        emit(Qd, A.perm, bd.first + lane, (int)S.x + 32768, ovf);
+        emit(Qd, A.perm, bd.first + kWave + lane, (int)S.y + 32768, ovf);
     }
 }
 
@@ -459,7 +462,8 @@ constexpr int kGroupStepRows = group_step_rows(WL, WG);
 
 template <int W, int NR>
 __device__ __forceinline__ void group_item(const ScanArgs& A, const QueryDesc& Qd, int local,
-                                           short* lds, uint2* slot, uint32_t (*s_red)[kWave]) {
+                                           short* lds, uint2* slot, uint32_t (*s_red)[kWave],
+                                           int* ovf) {
     const int wave = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
     const s2 Q = {(short)A.gap_oe, (short)A.gap_oe};
@@ -603,8 +607,8 @@ __device__ __forceinline__ void group_item(const ScanArgs& A, const QueryDesc& Q
     if (has_pair && gl == 0) {
         s2 m = S;
         for (int k = 1; k < G; ++k) m = vmax(m, as_s2(s_red[wave][lane + k]));
-        emit(Qd, A.perm, bd.first, (int)m.x + 32768);
-        emit(Qd, A.perm, bd.first + 1, (int)m.y + 32768);
+        emit(Qd, A.perm, bd.first, (int)m.x + 32768, ovf);
+        emit(Qd, A.perm, bd.first + 1, (int)m.y + 32768, ovf);
     }
 }
 
@@ -632,31 +636,170 @@ __device__ __forceinline__ bool next_scan_item(const ScanArgs& A, int* s_buf, in
     return local >= 0;
 }
 
+__device__ __forceinline__ uint32_t sys_load(const uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+__device__ __forceinline__ uint64_t sys_load(const uint64_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// Live queue (LiveQueue): thread 0 draws a ticket and waits, sleeping between
+// polls, until the host has published it, closed the queue or the idle limit
+// has passed.  The published count and the closed flag are one word, so one
+// read sees both as the host last wrote them.
+__device__ __forceinline__ bool next_live_item(const ScanArgs& A, int* s_buf, int& local, int& q,
+                                               int& kind, uint32_t& ticket) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t t = atomicAdd(A.work_counter, 1u);
+        uint2 v = make_uint2(0u, 0xffffffffu);
+        const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+        int nap = 1;   // s_sleep 127 (about 3.4 us) per unit
+        for (;;) {
+            const uint64_t c = sys_load(A.live.ctl);
+            if (t < (uint32_t)c) {
+                const uint64_t e = sys_load(reinterpret_cast<const uint64_t*>(
+                    A.live.ring + (t & (A.live.cap - 1))));
+                v = make_uint2((uint32_t)e, (uint32_t)(e >> 32));
+                break;
+            }
+            if ((c >> 32) != 0) break;   // closed, and t was not published
+            if (__builtin_amdgcn_s_memrealtime() - t0 > A.live.idle_ticks) {
+                __hip_atomic_store(A.live.status, 1u, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_SYSTEM);
+                break;
+            }
+            for (int k = 0; k < nap; ++k) __builtin_amdgcn_s_sleep(127);
+            nap = min(2 * nap, 32);
+        }
+        s_buf[0] = (int)v.y;
+        s_buf[1] = (int)(v.x & ~(kLaneItem | kPrepItem));
+        s_buf[2] = (v.x & kPrepItem) ? 2 : (v.x & kLaneItem) ? 1 : 0;
+        s_buf[3] = 0;   // the item's overflow flag (emit sets it)
+        s_buf[4] = (int)t;
+    }
+    __syncthreads();
+    local = __builtin_amdgcn_readfirstlane(s_buf[0]);
+    q = __builtin_amdgcn_readfirstlane(s_buf[1]);
+    kind = __builtin_amdgcn_readfirstlane(s_buf[2]);
+    ticket = (uint32_t)__builtin_amdgcn_readfirstlane(s_buf[4]);
+    return local >= 0;
+}
+
+// Prep item of task q: its descriptor block and profile from the page-locked
+// arena to the device arena, 16 bytes a thread per step (the profile's size,
+// 24 x pstride int16 with pstride a multiple of 96, is a multiple of 16).
+__device__ __forceinline__ void prep_item(const ScanArgs& A, int q) {
+    const size_t doff = (size_t)q * kLiveDescStride * sizeof(QueryDesc);
+    const QueryDesc* hd = reinterpret_cast<const QueryDesc*>(A.live.host_arena + doff);
+    const size_t poff = (size_t)(reinterpret_cast<const char*>(hd->prof) - A.live.dev_arena);
+    const size_t pbytes = (size_t)kProfRows * hd->pstride * sizeof(int16_t);
+    const uint4* src = reinterpret_cast<const uint4*>(A.live.host_arena + poff);
+    uint4* dst = reinterpret_cast<uint4*>(A.live.dev_arena + poff);
+    const int n = (int)(pbytes / sizeof(uint4));
+    constexpr int kUnroll = 4;
+    for (int i = threadIdx.x; i < n; i += kUnroll * 256) {
+        uint4 v[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u)
+            if (i + u * 256 < n) v[u] = src[i + u * 256];
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u)
+            if (i + u * 256 < n) dst[i + u * 256] = v[u];
+    }
+    constexpr int kDescVecs = kLiveDescStride * sizeof(QueryDesc) / sizeof(uint4);
+    if (threadIdx.x < kDescVecs)
+        reinterpret_cast<uint4*>(A.live.dev_arena + doff)[threadIdx.x] =
+            reinterpret_cast<const uint4*>(A.live.host_arena + doff)[threadIdx.x];
+}
+
+// End of a live item: every wave's scores and overflow entries are written
+// back to memory (a system-scope release: the L2 is not coherent with the
+// copy engine that reads them), then thread 0 reports the ticket done.
+__device__ __forceinline__ void finish_live_item(const ScanArgs& A, const int* s_buf,
+                                                 uint32_t ticket) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t v = ((ticket + 1u) << 1) | (s_buf[3] ? 1u : 0u);
+        __hip_atomic_store(A.live.done + (ticket & (A.live.cap - 1)), v, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+}
+
 // WL = 16: at most 168 VGPRs, so that three workgroups (12 waves) share a CU
-// as its LDS allows; wider strips run two.
-template <int WL, int WG, bool PAIR>
+// as its LDS allows; wider strips run two.  LIVE: items come from the live
+// queue (A.live) instead of the launch's item list.
+template <int WL, int WG, bool PAIR, bool LIVE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WL == 16 ? 3 : 2)))
 void k_scan(ScanArgs A) {
     static_assert(!PAIR || WL == 16 || WL == 24, "pair table only for W = 16, 24");
     extern __shared__ __align__(16) short lds[];
-    __shared__ int s_buf[3];
+    __shared__ int s_buf[5];
     __shared__ uint32_t s_red[kWavesPerWG][kWave];
     const int wave = threadIdx.x >> 6;
     uint2* slot = A.border + (size_t)(blockIdx.x * kWavesPerWG + wave) * A.border_stride;
     int loaded_q = -1, loaded_seg = -1;
     int local, q;
     bool is_group;
-    while (next_scan_item(A, s_buf, local, q, is_group)) {
-        if (is_group) {
-            const QueryDesc Qd = A.group_q[q];
-            group_item<WG, kGroupStepRows<WL, WG>>(A, Qd, local, lds, slot, s_red);
+    int kind = 0;   // live: 0 group, 1 lane, 2 prep item
+    uint32_t ticket = 0;
+    while (LIVE ? next_live_item(A, s_buf, local, q, kind, ticket)
+                : next_scan_item(A, s_buf, local, q, is_group)) {
+        const size_t dq = (size_t)q * A.desc_stride;
+        if constexpr (LIVE) is_group = kind == 0;
+        if (LIVE && kind == 2) {
+            prep_item(A, q);
+        } else if (is_group) {
+            const QueryDesc Qd = A.group_q[dq];
+            group_item<WG, kGroupStepRows<WL, WG>>(A, Qd, local, lds, slot, s_red, s_buf + 3);
             loaded_q = -1;   // the group item restaged the profile
         } else {
-            const QueryDesc Qd = A.lane_q[q];
-            lane_item<WL, PAIR>(A, Qd, q, local, lds, slot, loaded_q, loaded_seg);
+            const QueryDesc Qd = A.lane_q[dq];
+            lane_item<WL, PAIR>(A, Qd, q, local, lds, slot, loaded_q, loaded_seg, s_buf + 3);
         }
+        if constexpr (LIVE) finish_live_item(A, s_buf, ticket);
     }
 }
+
+#define SWH_SCAN_CASES(X)                                                                  \
+    X(16, 8, true) X(16, 16, true) X(24, 8, true) X(24, 16, true) X(16, 8, false)          \
+    X(16, 16, false) X(24, 8, false) X(24, 16, false) X(32, 8, false) X(32, 16, false)     \
+    X(24, 24, true) X(24, 24, false)
+
+template <typename K>
+static int occ(K kern, size_t lds) {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kern, 256, lds) != hipSuccess) return 1;
+    return n > 0 ? n : 1;
+}
+
+template <int WL, int WG, bool PAIR, bool LIVE>
+static hipError_t launch_scan_w(const ScanArgs& a, int grid, size_t lds, hipStream_t s) {
+    hipLaunchKernelGGL((k_scan<WL, WG, PAIR, LIVE>), dim3(grid), dim3(256), lds, s, a);
+    return hipGetLastError();
+}
+
+#ifdef SWH_LIVE_ONLY
+// The live-queue instantiations of the scan (swh_kernels_live.hip compiles
+// this file with SWH_LIVE_ONLY, so both halves build in parallel).
+hipError_t launch_scan_live(int wl, int wg, bool pair, const ScanArgs& a, int grid, size_t lds,
+                            hipStream_t s) {
+#define X(L, G, P) \
+    if (wl == L && wg == G && pair == P) return launch_scan_w<L, G, P, true>(a, grid, lds, s);
+    SWH_SCAN_CASES(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+int scan_live_max_wg_per_cu(int wl, int wg, bool pair, size_t lds) {
+#define X(L, G, P) \
+    if (wl == L && wg == G && pair == P) return occ(k_scan<L, G, P, true>, lds);
+    SWH_SCAN_CASES(X)
+#undef X
+    return 1;
+}
+#else
 
 // ---------------------------------------------------------------------------
 // int32 rescore of the overflowed sequences, in the pipeline of the group
@@ -932,21 +1075,10 @@ hipError_t launch_pack(const BlockDesc* blocks, int n_blocks, int n_pairs, const
     return hipGetLastError();
 }
 
-template <int WL, int WG, bool PAIR>
-static hipError_t launch_scan_w(const ScanArgs& a, int grid, size_t lds, hipStream_t s) {
-    hipLaunchKernelGGL((k_scan<WL, WG, PAIR>), dim3(grid), dim3(256), lds, s, a);
-    return hipGetLastError();
-}
-
-#define SWH_SCAN_CASES(X)                                                                  \
-    X(16, 8, true) X(16, 16, true) X(24, 8, true) X(24, 16, true) X(16, 8, false)          \
-    X(16, 16, false) X(24, 8, false) X(24, 16, false) X(32, 8, false) X(32, 16, false)     \
-    X(24, 24, true) X(24, 24, false)
-
 hipError_t launch_scan(int wl, int wg, bool pair, const ScanArgs& a, int grid, size_t lds,
                        hipStream_t s) {
 #define X(L, G, P) \
-    if (wl == L && wg == G && pair == P) return launch_scan_w<L, G, P>(a, grid, lds, s);
+    if (wl == L && wg == G && pair == P) return launch_scan_w<L, G, P, false>(a, grid, lds, s);
     SWH_SCAN_CASES(X)
 #undef X
     return hipErrorInvalidValue;
@@ -957,16 +1089,9 @@ hipError_t launch_rescore(const RescoreArgs& a, int grid, size_t lds, hipStream_
     return hipGetLastError();
 }
 
-template <typename K>
-static int occ(K kern, size_t lds) {
-    int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kern, 256, lds) != hipSuccess) return 1;
-    return n > 0 ? n : 1;
-}
-
 int scan_max_wg_per_cu(int wl, int wg, bool pair, size_t lds) {
 #define X(L, G, P) \
-    if (wl == L && wg == G && pair == P) return occ(k_scan<L, G, P>, lds);
+    if (wl == L && wg == G && pair == P) return occ(k_scan<L, G, P, false>, lds);
     SWH_SCAN_CASES(X)
 #undef X
     return 1;
@@ -979,5 +1104,6 @@ size_t lane_lds_bytes(int width, bool pair, int ps_lane) {
     if (pair && width == 24) return PairTab<24>::bytes;
     return (size_t)kProfRows * ps_lane * sizeof(int16_t);
 }
+#endif  // SWH_LIVE_ONLY
 
 }  // namespace swh

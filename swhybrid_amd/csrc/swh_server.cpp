@@ -9,7 +9,21 @@
 // on its query and the database: each task scans its own block ranges and
 // writes its own buffers, whatever else shares the launch.
 //
-// Slices.  By default a round holds about round_ms of device work: each
+// Live queue (the default on a database of one layout).  Instead of rounds,
+// one persistent scan launch stays on the device while there is work, and
+// the loop feeds it (swh_live.h): a request admitted while others run gets its
+// task at once and its items are published between theirs, shortest
+// remaining work first (with aging), keeping a short lead of queued items
+// ahead of the workgroups.  The kernel writes each request's scores straight
+// into its page-locked score copy, so a request whose items are all done is
+// complete while the launch scans the others, and the loop makes no call
+// while the launch runs that could wait for it (a copy that needs the CUs
+// the launch holds would).  The launch ends when nothing is left to publish
+// (or a request overflowed int16: the int32 rescore runs when it has ended),
+// and the next one starts when work arrives.
+//
+// Rounds.  With opts.queue = SWH_QUEUE_ROUNDS (and always on a small database
+// of two layouts) the loop runs rounds.  By default a round holds about round_ms of device work: each
 // request keeps a cursor over the database's pair and lane blocks, requests
 // are served shortest remaining work first (with aging, so none waits
 // forever), and each one's slice covers the same share of its remaining pair
@@ -24,6 +38,7 @@
 #include <unordered_map>
 
 #include "swh_host.h"
+#include "swh_live.h"
 #include "swh_plan.h"
 
 using namespace swhh;
@@ -46,6 +61,15 @@ struct Request {
     std::vector<int64_t> top_index;
     std::vector<int32_t> scores;      // copy of the score vector if asked for
     bool done = false;
+    // live queue: the request's items in launch order and how far they went
+    std::vector<uint2> items;
+    std::vector<double> rest;         // rest[i]: padded cells of items [i, end)
+    int group = 0;                    // lanes per group of its group items
+    size_t next_item = 0, items_done = 0;
+    int task = -1;                    // its task in the running launch
+    bool ready = false;               // its task's prep item is done
+    bool overflowed = false;          // an item of the running launch overflowed
+    bool scanned = false;             // every item done
 };
 
 }  // namespace
@@ -80,6 +104,10 @@ struct swh_server {
     std::string loop_error;
     swh_server_stats stats{};
     std::thread loop, completer;
+    // live queue
+    bool live = false;
+    std::unique_ptr<LiveScan> lq;
+    int32_t* d_scores = nullptr;        // device address of h_scores
 };
 
 namespace {
@@ -142,6 +170,33 @@ void plan_round(swh_server* s, const std::vector<std::shared_ptr<Request>>& batc
 // server is stopping), or a device failure (loop_error says which).
 enum class Round { kDone, kStop, kFail };
 
+// Admit waiting requests in arrival order while a slot is free and their
+// device bytes fit the budget (the first always fits when nothing is active).
+// Caller holds s->mu.
+void admit(swh_server* s, std::vector<std::shared_ptr<Request>>* admitted) {
+    while (!s->stop && !s->waiting.empty() && !s->free_slots.empty()) {
+        auto& r = s->waiting.front();
+        if (s->bytes_in_use + r->bytes > s->opts.memory_budget && !s->active.empty()) break;
+        r->slot = s->free_slots.back();
+        s->free_slots.pop_back();
+        s->bytes_in_use += r->bytes;
+        r->info.admit_ms = since(s);
+        s->active.push_back(r);
+        if (admitted) admitted->push_back(r);
+        s->waiting.pop_front();
+    }
+    s->stats.peak_inflight = std::max<int>(s->stats.peak_inflight, (int)s->active.size());
+    s->stats.waiting = (int)s->waiting.size();
+}
+
+// A request whose scores are in its pinned copy goes to the completion
+// thread.  Caller holds s->mu.
+void hand_over(swh_server* s, const std::shared_ptr<Request>& r) {
+    s->active.erase(std::find(s->active.begin(), s->active.end(), r));
+    s->to_complete.push_back(r);
+    s->cv_complete.notify_one();
+}
+
 // One round: admit, launch the next slice of every active request, wait.
 Round run_round(swh_server* s) {
     swh_ctx* c = s->ctx;
@@ -156,21 +211,9 @@ Round run_round(swh_server* s) {
             return s->stop || !s->active.empty() ||
                    (!s->waiting.empty() && !s->free_slots.empty());
         });
-        while (!s->stop && !s->waiting.empty() && !s->free_slots.empty()) {
-            auto& r = s->waiting.front();
-            if (s->bytes_in_use + r->bytes > s->opts.memory_budget && !s->active.empty()) break;
-            r->slot = s->free_slots.back();
-            s->free_slots.pop_back();
-            s->bytes_in_use += r->bytes;
-            r->info.admit_ms = since(s);
-            s->active.push_back(r);
-            admitted.push_back(r);
-            s->waiting.pop_front();
-        }
+        admit(s, &admitted);
         if (s->active.empty()) return s->stop ? Round::kStop : Round::kDone;
         batch = s->active;
-        s->stats.peak_inflight = std::max<int>(s->stats.peak_inflight, (int)batch.size());
-        s->stats.waiting = (int)s->waiting.size();
     }
     const double t_round = now_ms();
     (void)hipSetDevice(c->device);
@@ -271,17 +314,200 @@ Round run_round(swh_server* s) {
         r.info.cells += (uint64_t)r.query.size() * (range_residues(db, T.pair_begin, T.pair_end) +
                                                     range_residues(db, T.lane_begin, T.lane_end));
     }
-    for (auto& r : finished) {
-        s->active.erase(std::find(s->active.begin(), s->active.end(), r));
-        s->to_complete.push_back(r);
+    for (auto& r : finished) hand_over(s, r);
+    return Round::kDone;
+}
+
+// ---------------------------------------------------------------------------
+// Live queue.
+
+// Items queued ahead of the workgroups: enough that none waits for the loop
+// between two polls, few enough that a request arriving now starts soon.
+uint32_t live_lead(const LiveScan& L) { return (uint32_t)std::max(64, L.grid() / 2); }
+constexpr int kLivePublishChunk = 32;          // items of one request per pick
+constexpr auto kLivePoll = std::chrono::microseconds(100);
+constexpr double kLiveCloseWaitMs = 120000.0;  // a launch ends within its longest item
+
+// Profile and items of a newly admitted request.
+void plan_live_request(swh_server* s, Request& r) {
+    const int qlen = (int)r.query.size();
+    int16_t* hp = (int16_t*)s->h_prof.p + (size_t)r.slot * s->prof_slot_elems;
+    build_profile(s->ctx->matrix, r.query.data(), qlen, hp);
+    r.group = s->lq->group_lanes_of(qlen);
+    std::vector<double> cells;
+    s->lq->plan_items(qlen, r.group, r.items, cells);
+    r.rest.assign(cells.size() + 1, 0.0);
+    for (size_t i = cells.size(); i-- > 0;) r.rest[i] = r.rest[i + 1] + cells[i];
+    r.next_item = r.items_done = 0;
+}
+
+// A request whose scores are final goes to the completion thread.
+void finish_live(swh_server* s, const std::shared_ptr<Request>& r) {
+    r->info.cells = (uint64_t)r->query.size() * s->db->residues;
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stats.cells += r->info.cells;
+    hand_over(s, r);
+}
+
+// One live launch: from the first admitted request until nothing is left to
+// publish (or a rescore is due), feeding the kernel while it runs.
+Round run_epoch(swh_server* s) {
+    swh_ctx* c = s->ctx;
+    LiveScan& L = *s->lq;
+    {
+        std::unique_lock<std::mutex> lk(s->mu);
+        s->cv_work.wait(lk, [&] {
+            return s->stop || !s->active.empty() ||
+                   (!s->waiting.empty() && !s->free_slots.empty());
+        });
+        admit(s, nullptr);
+        if (s->active.empty()) return s->stop ? Round::kStop : Round::kDone;
     }
-    if (!finished.empty()) s->cv_complete.notify_one();
+    (void)hipSetDevice(c->device);
+    const double t_epoch = now_ms();
+    if (L.begin() != SWH_OK) {
+        s->loop_error = g_err;
+        return Round::kFail;
+    }
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stats.rounds += 1;   // launches begun (a request may finish before its launch ends)
+    }
+    std::vector<std::shared_ptr<Request>> mine;   // requests with a task in this launch
+    bool drain = false;                           // publish nothing more: end the launch
+    double padded_done = 0;
+    auto fail_epoch = [&](const std::string& why) {
+        s->loop_error = why;
+        (void)L.close_and_wait(kLiveCloseWaitMs);
+        return Round::kFail;
+    };
+    const uint32_t target = (uint32_t)L.grid() + live_lead(L);
+    for (;;) {
+        // admission; a task in this launch for every request without one
+        std::vector<std::shared_ptr<Request>> need;
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            admit(s, nullptr);
+            for (auto& r : s->active)
+                if (r->task < 0 && !r->scanned) need.push_back(r);
+        }
+        for (auto& r : need) {
+            if (drain || L.room() == 0) break;
+            if (r->items.empty()) plan_live_request(s, *r);
+            const int16_t* hp = (const int16_t*)s->h_prof.p + (size_t)r->slot * s->prof_slot_elems;
+            LiveTask t{hp, (int)r->query.size(), r->group, s->d_scores + (size_t)r->slot * s->db->nseq,
+                       (uint32_t*)s->ovf.p + (size_t)r->slot * s->max_slice_seqs};
+            const int q = L.add_task(t);
+            if (q < 0) {   // this launch's arena is full: the next launch takes it
+                drain = true;
+                break;
+            }
+            r->task = q;
+            r->ready = false;
+            r->info.rounds += 1;
+            mine.push_back(r);
+            L.publish(q, make_uint2(kPrepItem, 0), r.get());   // stages it in HBM
+        }
+        L.flush();
+        // completed items
+        L.poll([&](void* tag, uint32_t kind, bool ovf) {
+            Request& r = *(Request*)tag;
+            if (kind & kPrepItem) {
+                r.ready = true;
+                return;
+            }
+            ++r.items_done;
+            if (ovf) r.overflowed = true;
+        });
+        // (the kernel writes scores straight into the request's page-locked
+        // copy: a request whose items are done is complete, with no copy
+        // that would have to wait for the launch)
+        for (auto& r : mine) {
+            if (r->scanned || r->items_done < r->items.size()) continue;
+            r->scanned = true;
+            if (r->overflowed) drain = true;   // rescored once the launch has ended
+            else finish_live(s, r);
+        }
+        // publish: shortest remaining work first, with aging
+        if (!drain) {
+            const double now = since(s);
+            while (L.outstanding() < target && L.room() > 0) {
+                Request* best = nullptr;
+                double best_key = 0;
+                for (auto& r : mine) {
+                    if (!r->ready || r->next_item >= r->items.size()) continue;
+                    const double key = r->rest[r->next_item] -
+                                       s->opts.wait_weight * (now - r->info.submit_ms) * s->rate;
+                    if (!best || key < best_key) {
+                        best = r.get();
+                        best_key = key;
+                    }
+                }
+                if (!best) break;
+                const uint32_t n = std::min<uint32_t>(
+                    {(uint32_t)kLivePublishChunk, target - L.outstanding(), L.room(),
+                     (uint32_t)(best->items.size() - best->next_item)});
+                for (uint32_t k = 0; k < n; ++k) {
+                    padded_done += best->rest[best->next_item] - best->rest[best->next_item + 1];
+                    L.publish(best->task, best->items[best->next_item++], best);
+                }
+            }
+            L.flush();
+        }
+        if (L.failed()) return fail_epoch(g_err);
+        // the launch ends once nothing it holds is left and nothing can join
+        if (L.outstanding() == 0) {
+            bool more = false;
+            for (auto& r : mine)
+                if (r->next_item < r->items.size()) more = true;
+            if (!drain && !more) {
+                // (a waiting request that the budget keeps out waits for
+                // bytes that completions free: not a reason to stay)
+                std::lock_guard<std::mutex> lk(s->mu);
+                for (auto& r : s->active)
+                    if (r->task < 0 && !r->scanned) more = true;
+                if (!s->stop && !s->waiting.empty() && !s->free_slots.empty() &&
+                    (s->active.empty() ||
+                     s->bytes_in_use + s->waiting.front()->bytes <= s->opts.memory_budget))
+                    more = true;
+            }
+            if (drain || !more) break;
+        }
+        std::unique_lock<std::mutex> lk(s->mu);
+        s->cv_work.wait_for(lk, kLivePoll);
+    }
+    if (L.close_and_wait(kLiveCloseWaitMs) != SWH_OK) return fail_epoch(g_err);
+    // int32 rescore of what overflowed, then those requests' scores
+    std::vector<int> tasks;
+    for (auto& r : mine)
+        if (r->overflowed) tasks.push_back(r->task);
+    if (!tasks.empty()) {
+        if (L.rescore(tasks.data(), (int)tasks.size()) != SWH_OK ||
+            hipStreamSynchronize(c->stream) != hipSuccess) {
+            s->loop_error = "int32 rescore failed on the device";
+            return Round::kFail;
+        }
+        for (auto& r : mine) {
+            if (!r->overflowed) continue;
+            r->overflowed = false;
+            if (r->scanned) finish_live(s, r);   // else the overflowed ones are exact now
+        }
+    }
+    for (auto& r : mine) {
+        r->task = -1;
+        r->ready = false;
+    }
+    const double ms = now_ms() - t_epoch;
+    if (ms > 0 && padded_done > 0)
+        s->rate = std::min(2.0e10, std::max(1.0e8, 0.7 * s->rate + 0.3 * padded_done / ms));
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stats.busy_ms += ms;
     return Round::kDone;
 }
 
 void loop_main(swh_server* s) {
     for (;;) {
-        const Round r = run_round(s);
+        const Round r = s->live ? run_epoch(s) : run_round(s);
         const bool ok = r != Round::kFail;
         std::lock_guard<std::mutex> lk(s->mu);
         if (!ok || (s->stop && s->active.empty())) {
@@ -356,11 +582,16 @@ int swh_server_create(swh_ctx* c, swh_db* db, const swh_server_opts* opts, swh_s
     if (o.round_ms == 0) o.round_ms = kDefaultRoundMs;
     if (o.wait_weight == 0) o.wait_weight = kDefaultWaitWeight;
     if (o.max_inflight < 1 || o.slices < 0 || o.max_qlen < 1 || o.max_k < 1 ||
-        !(o.round_ms > 0) || !(o.wait_weight >= 0))
+        !(o.round_ms > 0) || !(o.wait_weight >= 0) || o.queue < SWH_QUEUE_AUTO ||
+        o.queue > SWH_QUEUE_ROUNDS)
         return fail(SWH_ERR_ARG, "bad server options");
-    // budgeted rounds on a database of one layout; a small database (two
-    // layouts, a whole round of a few milliseconds) runs whole rounds
-    const bool budgeted = o.slices == 0 && !db->alt;
+    // the live queue on a database of one layout, or when asked for (then on
+    // the first layout); otherwise rounds: budgeted on a database of one
+    // layout, whole rounds on a small database (two layouts, a whole round of
+    // a few milliseconds)
+    const bool live = o.queue == SWH_QUEUE_LIVE ||
+                      (o.queue == SWH_QUEUE_AUTO && o.slices == 0 && !db->alt);
+    const bool budgeted = !live && o.slices == 0 && !db->alt;
     if (o.slices == 0) o.slices = 1;
     int rc = check_opts(o.search);
     if (rc) return rc;
@@ -383,6 +614,7 @@ int swh_server_create(swh_ctx* c, swh_db* db, const swh_server_opts* opts, swh_s
     s->db = db;
     s->opts = o;
     s->budgeted = budgeted;
+    s->live = live;
     s->t0 = now_ms();
     s->pair_cut = cut_blocks(db, 0, db->n_pairs, o.slices);
     s->lane_cut = cut_blocks(db, db->n_pairs, db->n_blocks, o.slices);
@@ -391,7 +623,7 @@ int swh_server_create(swh_ctx* c, swh_db* db, const swh_server_opts* opts, swh_s
                          (size_t)(s->lane_cut[i + 1] - s->lane_cut[i]) * kLaneSeqs;
         s->max_slice_seqs = std::max(s->max_slice_seqs, n);
     }
-    if (budgeted)   // a slice may cover the whole database
+    if (budgeted || live)   // a slice (a task) may cover the whole first layout
         s->max_slice_seqs = (size_t)db->n_pairs * 2 +
                             (size_t)(db->n_blocks - db->n_pairs) * kLaneSeqs;
     if (o.slices == 1 && db->alt)   // a round may run on the second layout
@@ -406,7 +638,8 @@ int swh_server_create(swh_ctx* c, swh_db* db, const swh_server_opts* opts, swh_s
         1, std::min<uint64_t>((uint64_t)o.max_inflight, o.memory_budget / per_slot));
     s->prof_slot_elems = (size_t)kProfRows * profile_stride(o.max_qlen);
     const size_t nseq = std::max<size_t>(db->nseq, 1);
-    hipError_t e = s->scores.reserve((size_t)slots * nseq * sizeof(int32_t));
+    // (the live queue's kernel writes scores into the page-locked copies)
+    hipError_t e = live ? hipSuccess : s->scores.reserve((size_t)slots * nseq * sizeof(int32_t));
     if (e == hipSuccess) e = s->ovf.reserve((size_t)slots * s->max_slice_seqs * sizeof(uint32_t));
     if (e == hipSuccess) e = s->prof.reserve((size_t)slots * s->prof_slot_elems * sizeof(int16_t));
     if (e == hipSuccess) e = s->counters.reserve((16 + (size_t)slots) * sizeof(uint32_t));
@@ -421,6 +654,41 @@ int swh_server_create(swh_ctx* c, swh_db* db, const swh_server_opts* opts, swh_s
         s->h_scores.release();
         detach();
         return fail(SWH_ERR_NOMEM, "server buffers: %s", hipGetErrorString(e));
+    }
+    if (live && (size_t)kProfRows * profile_stride(o.max_qlen) * sizeof(int16_t) >
+                    LiveScan::kArenaBytes) {
+        s->scores.release();
+        s->ovf.release();
+        s->prof.release();
+        s->counters.release();
+        s->h_prof.release();
+        s->h_scores.release();
+        detach();
+        return fail(SWH_ERR_ARG, "max_qlen %d: its profile exceeds the live queue's %zu MB arena",
+                    o.max_qlen, LiveScan::kArenaBytes >> 20);
+    }
+    if (live) {
+        // one launch shape for every request: the one a 512-residue query
+        // (within the caller's strip and group options) would run with
+        const int rep = std::min(512, o.max_qlen);
+        const Shape sh = choose_shape(c, &rep, 1, o.search, db, db->residues);
+        s->lq = std::make_unique<LiveScan>();
+        rc = s->lq->init(c, db, sh, o.search);
+        if (rc == SWH_OK && hipHostGetDevicePointer((void**)&s->d_scores, s->h_scores.p, 0) !=
+                                hipSuccess)
+            rc = fail(SWH_ERR_HIP, "no device address for the score copies");
+        if (rc != SWH_OK) {
+            const std::string why = g_err;
+            s->lq.reset();
+            s->scores.release();
+            s->ovf.release();
+            s->prof.release();
+            s->counters.release();
+            s->h_prof.release();
+            s->h_scores.release();
+            detach();
+            return fail(rc, "%s", why.c_str());
+        }
     }
     for (int i = slots - 1; i >= 0; --i) s->free_slots.push_back(i);
     swh_server* raw = s.release();
@@ -521,6 +789,7 @@ int swh_server_destroy(swh_server* s) {
         s->cv_done.wait(lk, [&] { return s->waiters == 0; });
     }
     (void)hipSetDevice(s->ctx->device);
+    s->lq.reset();
     s->scores.release();
     s->ovf.release();
     s->prof.release();

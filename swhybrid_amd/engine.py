@@ -193,10 +193,14 @@ class Server:
     blocks for its top-k (and optionally its full score vector)."""
 
     def __init__(self, engine: Engine, db: Database, max_inflight=0, memory_budget=0,
-                 slices=0, max_qlen=0, max_k=0, round_ms=0.0, wait_weight=0.0,
+                 slices=0, max_qlen=0, max_k=0, round_ms=0.0, wait_weight=0.0, queue="auto",
                  **search_opts):
+        """queue: "live" (one persistent launch fed while it runs, the
+        default on a database of one layout), "rounds" (one launch per round;
+        slices / round_ms shape them) or "auto"."""
         o = N.SwhServerOpts(int(max_inflight), int(memory_budget), int(slices), int(max_qlen),
-                            int(max_k), _opts(**search_opts), float(round_ms), float(wait_weight))
+                            int(max_k), _opts(**search_opts), float(round_ms), float(wait_weight),
+                            N.QUEUE[queue])
         h = C.c_void_p()
         N.check(N.lib().swh_server_create(engine.handle, db.handle, C.byref(o), C.byref(h)))
         self.handle = h

@@ -136,7 +136,8 @@ def test_attached_server_refuses_other_calls(setup):
     eng.close()
 
 
-def test_destroy_fails_waiting_requests_with_state(setup):
+@pytest.mark.parametrize("queue", ["rounds", "live"])
+def test_destroy_fails_waiting_requests_with_state(setup, queue):
     """A budget that holds one request at a time: destroy finishes the one on
     the device and fails the waiting ones with SWH_ERR_STATE, also for a
     thread already blocked waiting on one of them."""
@@ -149,8 +150,9 @@ def test_destroy_fails_waiting_requests_with_state(setup):
     eng = Engine("blosum62", 10, 2)
     d = eng.load_db(db)
     qs = G.queries([6000] * 4, 96)
-    need = request_bytes(eng, d, qs[0], slices=4)
-    srv = Server(eng, d, memory_budget=need + need // 2, slices=4)
+    opts = dict(slices=4, queue="rounds") if queue == "rounds" else dict(queue="live")
+    need = request_bytes(eng, d, qs[0], **opts)
+    srv = Server(eng, d, memory_budget=need + need // 2, **opts)
     ids = [srv.submit(q, k=5) for q in qs]
     codes = {}
 
@@ -181,26 +183,35 @@ def test_request_above_budget_refused_at_submit(setup):
             srv.submit(queries(1, 97)[0], k=5)
 
 
-def test_budgeted_rounds_slice_and_stay_exact():
-    """Budgeted rounds (the default on a database of one layout): with 2 ms of
-    work per round, requests of 60-3000 residues submitted together are cut
-    into many slices of pair and lane blocks, shortest remaining work first.
-    Every score vector equals the request's search alone; the short requests
-    finish before the long ones; a sample equals the oracle."""
+@pytest.fixture(scope="module")
+def big():
+    """A database of one layout (more than 1,024 sequences per CU): 300k
+    sequences of log-normal length and 600 long ones (2,500-12,000)."""
     rng = np.random.Generator(np.random.PCG64(99))
     lens = np.concatenate([G.lognormal_lengths(rng, 300_000, 330, 0.7, 20, 2500),
                            G.loguniform_lengths(rng, 600, 2500, 12000)])
     db = G.SeqDB.from_lengths(rng, lens)
     eng = Engine("blosum62", 10, 2)
     d = eng.load_db(db)
+    yield eng, db, d
+    d.close()
+    eng.close()
+
+
+def test_budgeted_rounds_slice_and_stay_exact(big):
+    """Budgeted rounds: with 2 ms of work per round, requests of 60-3000
+    residues submitted together are cut into many slices of pair and lane
+    blocks, shortest remaining work first.  Every score vector equals the
+    request's search alone; the short requests finish before the long ones; a
+    sample equals the oracle."""
+    eng, db, d = big
+    rng = np.random.Generator(np.random.PCG64(101))
     qs = G.queries([3000, 60, 900, 250, 1800, 120], 100)
     alone = [eng.search(d, q)[0].copy() for q in qs]
-    with Server(eng, d, round_ms=2.0) as srv:
+    with Server(eng, d, round_ms=2.0, queue="rounds") as srv:
         ids = [srv.submit(q, k=10, want_scores=True) for q in qs]
         res = [srv.wait(i) for i in ids]
         st = srv.stats()
-    d.close()
-    eng.close()
     for i, (r, a) in enumerate(zip(res, alone)):
         assert np.array_equal(r["scores"], a), f"request {i} (length {len(qs[i])})"
         s, ix = topk(a, 10)
@@ -217,3 +228,120 @@ def test_budgeted_rounds_slice_and_stay_exact():
     for q, r in zip(qs[:3], res[:3]):
         want = O.search_i32(q, db.subset(idx), eng.matrix, 10, 2)
         assert np.array_equal(r["scores"][idx], want)
+
+
+# ---------------------------------------------------------------------------
+# Live queue: one persistent launch fed while it runs.
+
+def test_live_queue_requests_join_the_running_launch(big):
+    """A long request starts alone; shorter ones submitted while it runs are
+    appended to the same launch and, being shorter, finish first.  Every
+    score vector equals the request's search alone and a sample equals the
+    oracle."""
+    eng, db, d = big
+    rng = np.random.Generator(np.random.PCG64(102))
+    qs = G.queries([8000, 60, 900, 250, 1800, 120, 40, 2500], 103)
+    alone = [eng.search(d, q)[0].copy() for q in qs]
+    with Server(eng, d, queue="live", max_qlen=8192) as srv:
+        ids = [srv.submit(qs[0], k=10, want_scores=True)]
+        time.sleep(0.02)
+        for q in qs[1:]:
+            ids.append(srv.submit(q, k=10, want_scores=True))
+            time.sleep(0.002)
+        res = [srv.wait(i) for i in ids]
+        st = srv.stats()
+    for i, (r, a) in enumerate(zip(res, alone)):
+        assert np.array_equal(r["scores"], a), f"request {i} (length {len(qs[i])})"
+        s, ix = topk(a, 10)
+        assert np.array_equal(r["top_score"], s) and np.array_equal(r["top_index"], ix)
+        assert r["info"]["status"] == 0 and r["info"]["cells"] == len(qs[i]) * db.total
+    done = [r["info"]["done_ms"] for r in res]
+    admit = [r["info"]["admit_ms"] for r in res]
+    # the short ones were admitted while the long one ran and finished before it
+    for i in (1, 5, 6):
+        assert admit[i] < done[0] and done[i] < done[0], (i, admit, done)
+    assert st["completed"] == len(qs) and st["peak_inflight"] >= 2
+    assert st["cells"] == db.total * sum(len(q) for q in qs)
+    idx = np.unique(np.concatenate([np.arange(db.n - 50, db.n),
+                                    rng.choice(db.n, 300, replace=False)]))
+    for i in (1, 3, 6):
+        want = O.search_i32(qs[i], db.subset(idx), eng.matrix, 10, 2)
+        assert np.array_equal(res[i]["scores"][idx], want)
+
+
+def test_live_queue_alone_equals_in_the_mix(setup):
+    """On a small database (served on its first layout): a request alone and
+    among others submitted from several threads gives the same scores."""
+    eng, db, d = setup
+    qs = queries(12, 104)
+    target = qs[0]
+    want = O.search_i32(target, db, eng.matrix, 10, 2)
+    with Server(eng, d, queue="live") as srv:
+        alone = srv.wait(srv.submit(target, k=20, want_scores=True))
+        assert np.array_equal(alone["scores"], want)
+        ids = {}
+
+        def sub(i):
+            ids[i] = srv.submit(qs[i], k=20, want_scores=True)
+        ts = [threading.Thread(target=sub, args=(i,)) for i in range(len(qs))]
+        for t in ts:
+            t.start()
+        for t in ts:
+            t.join()
+        res = {i: srv.wait(ids[i]) for i in ids}
+        st = srv.stats()
+    assert np.array_equal(res[0]["scores"], want)
+    for i, q in enumerate(qs):
+        w = O.search_i32(q, db, eng.matrix, 10, 2)
+        assert np.array_equal(res[i]["scores"], w), i
+        s, ix = O.topk(w, 20)
+        assert np.array_equal(res[i]["top_score"], s), i
+        assert np.array_equal(res[i]["top_index"], ix), i
+    assert st["completed"] == len(qs) + 1
+
+
+def test_live_queue_forced_rescore(setup):
+    """An overflow threshold of 40 sends most sequences to the int32 rescore,
+    which runs when the launch has ended: scores stay exact."""
+    eng, db, d = setup
+    qs = queries(4, 105)
+    with Server(eng, d, queue="live", overflow_threshold=40) as srv:
+        ids = [srv.submit(q, k=10, want_scores=True) for q in qs]
+        res = [srv.wait(i) for i in ids]
+    for q, r in zip(qs, res):
+        assert np.array_equal(r["scores"], O.search_i32(q, db, eng.matrix, 10, 2))
+
+
+def test_live_queue_memory_admission_serialises(setup):
+    """A budget that holds one request: the others wait for its memory and
+    join the launch (or the next one) when it is freed."""
+    eng, db, d = setup
+    rng = np.random.Generator(np.random.PCG64(106))
+    qs = [G.random_residues(rng, 400) for _ in range(5)]
+    need = request_bytes(eng, d, qs[0], queue="live")
+    with Server(eng, d, memory_budget=need + 1, queue="live") as srv:
+        ids = [srv.submit(q, k=5) for q in qs]
+        res = [srv.wait(i) for i in ids]
+        st = srv.stats()
+    assert st["peak_inflight"] == 1
+    for q, r in zip(qs, res):
+        s, ix = O.topk(O.search_i32(q, db, eng.matrix, 10, 2), 5)
+        assert np.array_equal(r["top_score"], s) and np.array_equal(r["top_index"], ix)
+
+
+def test_live_queue_arena_overflows_into_the_next_launch(setup):
+    """200 requests of 8,000 residues: their profiles (387 KB each) exceed one
+    launch's 64 MB arena, so later requests run in later launches whose
+    arena reuses the same page-locked addresses.  Every top-k equals the
+    request's search alone."""
+    eng, db, d = setup
+    rng = np.random.Generator(np.random.PCG64(107))
+    qs = [G.random_residues(rng, 8000) for _ in range(200)]
+    alone = [topk(eng.search(d, q)[0], 5) for q in qs]
+    with Server(eng, d, queue="live", max_qlen=8192) as srv:
+        ids = [srv.submit(q, k=5) for q in qs]
+        res = [srv.wait(i) for i in ids]
+        st = srv.stats()
+    assert st["rounds"] >= 2, st
+    for i, (r, (s, ix)) in enumerate(zip(res, alone)):
+        assert np.array_equal(r["top_score"], s) and np.array_equal(r["top_index"], ix), i
