@@ -433,6 +433,23 @@ def run_serving(eng, d, qps_levels, n_requests, round_ms=0.0, slices=0, wait_wei
     from swhybrid_amd import datagen as G
     from swhybrid_amd.engine import Server
     res = {}
+    # burst: the 64 batched queries submitted at once (throughput of the loop
+    # at saturation, next to the one batched call)
+    qs = G.c3_queries()
+    with Server(eng, d, round_ms=round_ms, slices=slices, wait_weight=wait_weight,
+                queue=queue) as srv:
+        t0 = time.perf_counter()
+        ids = [srv.submit(q, k=TOP_K) for q in qs]
+        got = [srv.wait(i) for i in ids]
+        wall = time.perf_counter() - t0
+        st = srv.stats()
+    cells = sum(r["info"]["cells"] for r in got)
+    res["burst"] = {"workload": "the 64 batched queries submitted at once", "requests": len(qs),
+                    "gcups": round(cells / wall / 1e9, 1), "wall_s": round(wall, 3),
+                    "rounds": st["rounds"], "queue": queue,
+                    "first_done_ms": round(min(r["info"]["latency_ms"] for r in got), 1)}
+    log(f"serving burst: {len(qs)} requests in {wall:.2f} s = {res['burst']['gcups']} GCUPS "
+        f"({st['rounds']} rounds)")
     alone_ms = {}
     for qps in qps_levels:
         arrivals, qs = G.c4_requests(n_requests, qps)
