@@ -68,3 +68,35 @@ def test_host_topk_large_vectors_in_parts():
         a = topk(s, k)
         b = O.topk(s, k)
         assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]), (n, hi, k)
+
+
+def test_struct_mirrors_match_the_header(tmp_path):
+    """Every ctypes mirror of a struct in include/swh.h and
+    include/swhybrid_host.h has the C compiler's size and field offsets."""
+    import subprocess
+    mirrors = {"swh_stats": N.SwhStats, "swh_search_opts": N.SwhSearchOpts,
+               "swh_db_opts": N.SwhDbOpts, "swh_tune_entry": N.SwhTuneEntry,
+               "swh_tune_result": N.SwhTuneResult, "swh_server_opts": N.SwhServerOpts,
+               "swh_request_info": N.SwhRequestInfo, "swh_server_stats": N.SwhServerStats,
+               "swhy_pool_opts": N.SwhyPoolOpts, "swhy_pool_report": N.SwhyPoolReport}
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "swh.h"',
+             '#include "swhybrid_host.h"', "int main(void) {"]
+    for cname, py in mirrors.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f, _ in py._fields_:
+            lines.append(f'printf("{cname} {f} %zu\\n", offsetof({cname}, {f}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)],
+                   check=True)
+    got = {}
+    for line in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n"):
+        if line:
+            c, f, v = line.split()
+            got[(c, f)] = int(v)
+    for cname, py in mirrors.items():
+        assert got[(cname, "size")] == C.sizeof(py), cname
+        for f, _ in py._fields_:
+            assert got[(cname, f)] == getattr(py, f).offset, (cname, f)
