@@ -8,8 +8,8 @@
 // page-locked memory, so a task whose items are done is complete while the
 // kernel scans the others.  Tasks' descriptors and profiles are written by the
 // host into a page-locked arena and staged into a device arena by each task's
-// prep item (the GPU does not cache page-locked memory: scans reading it
-// there ran about 20 times slower); each launch fills both arenas from the
+// prep item, so scans read them from HBM, not across PCIe (every strip's pair
+// table reads the profile again); each launch fills both arenas from the
 // start, so no cache line the kernel has read can be stale.  Nothing the
 // kernel waits for depends on a copy engine or another launch: it only polls
 // host memory.
