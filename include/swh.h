@@ -108,7 +108,9 @@ int swh_pinned_free(void* p);
  * Param scoring fields it reads (src/Param.h:59-117). */
 int swh_ctx_create(int device, const int32_t* matrix, int gap_open, int gap_extend,
                    swh_ctx** out);
-/* While a server is attached to a context (swh_server_create), its loop
+/* swh_ctx_destroy returns SWH_ERR_STATE while databases made on the context
+ * still exist (swh_db_destroy them first).
+ * While a server is attached to a context (swh_server_create), its loop
  * thread owns the context's stream and launch buffers: swh_ctx_destroy,
  * swh_ctx_set_scoring, swh_search, swh_search_batch, swh_search_noalloc,
  * swh_autotune and swh_set_tuning on that context, and swh_db_destroy of the
@@ -129,7 +131,9 @@ int swh_db_create(swh_ctx* ctx, const uint8_t* residues, const uint64_t* offsets
  * the worker pipeline builds one chunk's database while the previous chunk is
  * searched.  No search on db may still be running. */
 int swh_db_destroy(swh_db* db);
-/* nseq, total residues, device bytes held by the packed layout. */
+/* nseq, total residues, device bytes the packed layout(s) use: the packed
+ * residues and the block, order and length tables (the buffers holding them
+ * may be larger: they are reused from destroyed databases). */
 int swh_db_info(const swh_db* db, uint64_t* nseq, uint64_t* residues, uint64_t* device_bytes);
 
 /* One query, one search: int16 scan + on-device int32 rescore; scores[i] for

@@ -106,6 +106,9 @@ int swh_ctx_destroy(swh_ctx* c) {
     if (!c) return SWH_OK;
     int rc = refuse_if_served(c, "swh_ctx_destroy");
     if (rc) return rc;
+    if (const int n = c->live_dbs.load())
+        return fail(SWH_ERR_STATE, "swh_ctx_destroy: %d database layout(s) made on this context "
+                    "still exist (swh_db_destroy them first)", n);
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->upload) (void)hipStreamSynchronize(c->upload);
@@ -113,6 +116,8 @@ int swh_ctx_destroy(swh_ctx* c) {
     c->cache.release();
     c->stage_raw.release();
     c->stage_soff.release();
+    c->h_stage.release();
+    free_retired(c);
     c->border.release();
     c->counters.release();
     c->prof.release();
@@ -177,6 +182,14 @@ static uint32_t default_group_min_len(const swh_ctx* c, const uint32_t* lengths,
 constexpr uint64_t kSmallDbSeqsPerCu = 1024;
 // Staging buffers up to this size stay with the context between uploads.
 constexpr size_t kStageKeepBytes = size_t(2) << 30;
+// Host-to-device copies of up to 16 KB are carried out by a copy kernel, which
+// waits for CUs: beside a search that holds every CU (the worker pipeline
+// builds chunk k + 1 while chunk k is searched) it starts only when that search
+// ends, and every copy queued after it on the stream waits too.  Copies of
+// 64 KB and more go to the copy engines and run beside the search (measured
+// on MI355X: 4 and 16 KB waited 156 ms for a 370 ms kernel, 64 KB to 4 MB did
+// not wait).  A database build therefore copies at least this much each time.
+constexpr size_t kMinEngineCopy = size_t(64) << 10;
 constexpr uint32_t kSmallDbGroupMinLen = 128;
 
 // At least one lane item (4 lane blocks) per workgroup slot, 2 slots per CU.
@@ -198,6 +211,59 @@ static bool is_pinned_host(const void* p) {
     return a.type == hipMemoryTypeHost;
 }
 
+namespace swhh {
+
+template <typename B>
+static hipError_t grow_retiring(swh_ctx* c, B& b, size_t want, bool exact,
+                                std::vector<B> swh_ctx::*list) {
+    if (want <= b.bytes) return hipSuccess;
+    if (b.fixed) return hipErrorOutOfMemory;
+    const B old = b;
+    b.p = nullptr;
+    b.bytes = 0;
+    const size_t n = exact ? want : std::max(want, old.bytes + old.bytes / 2);
+    hipError_t e = b.reserve(n);
+    if (e == hipErrorOutOfMemory) {   // the retired buffers may hold what is missing
+        (void)hipGetLastError();
+        free_retired(c);
+        e = b.reserve(n);
+    }
+    if (e != hipSuccess) {
+        b = old;
+        return e;
+    }
+    if (old.p) {
+        bool over;
+        {
+            std::lock_guard<std::mutex> g(c->retired_mu);
+            (c->*list).push_back(old);
+            c->retired_bytes += old.bytes;
+            over = c->retired_bytes > swh_ctx::kRetiredMax;
+        }
+        if (over) free_retired(c);
+    }
+    return hipSuccess;
+}
+
+hipError_t grow_beside(swh_ctx* c, DevBuf& b, size_t want, bool exact) {
+    return grow_retiring(c, b, want, exact, &swh_ctx::retired_dev);
+}
+
+hipError_t grow_beside(swh_ctx* c, PinnedBuf& b, size_t want, bool exact) {
+    return grow_retiring(c, b, want, exact, &swh_ctx::retired_host);
+}
+
+void free_retired(swh_ctx* c) {
+    std::lock_guard<std::mutex> g(c->retired_mu);
+    for (DevBuf& b : c->retired_dev) b.release();
+    for (PinnedBuf& b : c->retired_host) b.release();
+    c->retired_dev.clear();
+    c->retired_host.clear();
+    c->retired_bytes = 0;
+}
+
+}  // namespace swhh
+
 // One layout of the database: sequences of at least group_min residues in
 // pair blocks, the rest in lane blocks.
 static int create_layout(swh_ctx* c, const uint8_t* residues, const uint64_t* offsets,
@@ -208,6 +274,7 @@ static int create_layout(swh_ctx* c, const uint8_t* residues, const uint64_t* of
     static std::atomic<uint64_t> next_uid{1};
     auto* db = new swh_db();
     db->ctx = c;
+    c->live_dbs.fetch_add(1);
     db->uid = next_uid.fetch_add(1);
     db->nseq = nseq;
     db->group_min_len = group_min;
@@ -310,29 +377,59 @@ static int create_layout(swh_ctx* c, const uint8_t* residues, const uint64_t* of
         for (uint64_t i = 0; i < nseq; ++i) soff[i] = lengths[i] > 0 ? offsets[i] - lo : 0;
     }
     // device buffers come from the context's spares where one fits and the
-    // staging buffers are the context's own and only grow: no hipFree here
-    // (it would wait for every search running on the device)
-    auto up = [&](DevBuf& d, const void* h, size_t bytes) -> hipError_t {
-        hipError_t r = c->cache.take(d, std::max<size_t>(bytes, 16));
-        if (r != hipSuccess || bytes == 0) return r;
-        return hipMemcpy(d.p, h, bytes, hipMemcpyHostToDevice);
-    };
+    // staging buffers are the context's own and only grow, with slack: no
+    // hipFree here (it would wait for every search running on the device).
+    // Every table goes up from page-locked staging in one copy of at least
+    // kMinEngineCopy bytes (padded), on the upload stream.
     std::lock_guard<std::mutex> stage_lock(c->stage_mu);
+    auto padded = [](size_t bytes) {
+        return (std::max(bytes, kMinEngineCopy) + 4095) / 4096 * 4096;
+    };
+    // residues: page-locked ones (a worker's chunk) go up as they are; small
+    // pageable ones through the staging; large pageable ones (a database
+    // built once) as a prefix the runtime stages itself, in multiples of
+    // kMinEngineCopy, and a padded tail through the staging
+    const bool direct = stage_bytes >= kMinEngineCopy && is_pinned_host(stage);
+    const bool split = !direct && stage_bytes > (size_t(64) << 20);
+    const size_t head = split ? stage_bytes / kMinEngineCopy * kMinEngineCopy : 0;
+    const size_t tables[5] = {db->n_blocks * sizeof(BlockDesc), perm.size() * 4, slen.size() * 4,
+                              sblk.size() * 4, soff.size() * 8};
+    size_t h_need = direct ? 0 : padded(stage_bytes - head);
+    for (size_t t : tables) h_need += padded(t);
+    hipError_t e = grow_beside(c, c->h_stage, h_need + h_need / 4);
+    char* h_at = (char*)c->h_stage.p;
+    // bytes at h into d at offset at, from the staging, padded
+    auto up = [&](DevBuf& d, size_t at, const void* h, size_t bytes, bool cached) -> hipError_t {
+        const size_t n = padded(bytes);
+        hipError_t r = cached ? c->cache.take(d, at + n) : grow_beside(c, d, at + n + n / 4);
+        if (r != hipSuccess) return r;
+        if (bytes) std::memcpy(h_at, h, bytes);
+        r = hipMemcpyAsync((char*)d.p + at, h_at, n, hipMemcpyHostToDevice, c->upload);
+        h_at += n;
+        return r;
+    };
     DevBuf& raw = c->stage_raw;
     DevBuf& d_soff = c->stage_soff;
-    auto stage_up = [&](DevBuf& d, const void* h, size_t bytes) -> hipError_t {
-        hipError_t r = d.reserve_grow(std::max<size_t>(bytes, 16));
-        if (r != hipSuccess || bytes == 0) return r;
-        return hipMemcpy(d.p, h, bytes, hipMemcpyHostToDevice);
-    };
-    hipError_t e = c->cache.take(db->words, std::max<uint64_t>(woff, 1) * 4);
-    if (e == hipSuccess && db->n_blocks > 0)
-        e = up(db->d_blocks, db->blocks.data(), db->n_blocks * sizeof(BlockDesc));
-    if (e == hipSuccess) e = up(db->perm, perm.data(), perm.size() * 4);
-    if (e == hipSuccess) e = up(db->sorted_len, slen.data(), slen.size() * 4);
-    if (e == hipSuccess) e = up(db->sorted_blk, sblk.data(), sblk.size() * 4);
-    if (e == hipSuccess) e = stage_up(raw, stage, stage_bytes);
-    if (e == hipSuccess) e = stage_up(d_soff, soff.data(), soff.size() * 8);
+    db->dev_bytes = woff * 4 + tables[0] + tables[1] + tables[2] + tables[3];
+    if (e == hipSuccess) e = c->cache.take(db->words, std::max<uint64_t>(woff, 1) * 4);
+    if (e == hipSuccess) e = up(db->d_blocks, 0, db->blocks.data(), tables[0], true);
+    if (e == hipSuccess) e = up(db->perm, 0, perm.data(), tables[1], true);
+    if (e == hipSuccess) e = up(db->sorted_len, 0, slen.data(), tables[2], true);
+    if (e == hipSuccess) e = up(db->sorted_blk, 0, sblk.data(), tables[3], true);
+    if (e == hipSuccess) e = up(d_soff, 0, soff.data(), tables[4], false);
+    if (e == hipSuccess && direct) {
+        e = grow_beside(c, raw, stage_bytes + stage_bytes / 4);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(raw.p, stage, stage_bytes, hipMemcpyHostToDevice, c->upload);
+    } else if (e == hipSuccess) {
+        // sized once, as up() below sizes it: the prefix copy must land in the
+        // buffer the tail goes to
+        const size_t tail_n = padded(stage_bytes - head);
+        e = grow_beside(c, raw, head + tail_n + tail_n / 4);
+        if (e == hipSuccess && head)
+            e = hipMemcpyAsync(raw.p, stage, head, hipMemcpyHostToDevice, c->upload);
+        if (e == hipSuccess) e = up(raw, head, stage + head, stage_bytes - head, false);
+    }
     if (e == hipSuccess && db->n_blocks > 0)
         e = launch_pack((const BlockDesc*)db->d_blocks.p, db->n_blocks, np,
                         (const uint32_t*)db->perm.p, (const uint32_t*)db->sorted_len.p,
@@ -403,6 +500,7 @@ int swh_db_destroy(swh_db* db) {
         (void)hipSetDevice(db->ctx->device);
         for (DevBuf* b : {&db->words, &db->d_blocks, &db->perm, &db->sorted_len, &db->sorted_blk})
             db->ctx->cache.give(*b);
+        db->ctx->live_dbs.fetch_sub(1);
     }
     delete db;
     return SWH_OK;
@@ -414,9 +512,7 @@ int swh_db_info(const swh_db* db, uint64_t* nseq, uint64_t* residues, uint64_t* 
     if (residues) *residues = db->residues;
     if (device_bytes) {
         *device_bytes = 0;
-        for (const swh_db* d = db; d; d = d->alt)
-            *device_bytes += d->words.bytes + d->d_blocks.bytes + d->perm.bytes +
-                             d->sorted_len.bytes + d->sorted_blk.bytes;
+        for (const swh_db* d = db; d; d = d->alt) *device_bytes += d->dev_bytes;
     }
     return SWH_OK;
 }
@@ -799,8 +895,8 @@ int enqueue_tasks(swh_ctx* c, swh_db* db, const Task* t, int nt, const Shape& sh
         (lr.desc->fixed && total_bytes > lr.desc->bytes))
         return fail(SWH_ERR_CAPACITY, "%zu bytes of work descriptors, above the fixed %zu",
                     total_bytes, std::min(lr.h_desc->bytes, lr.desc->bytes));
-    HIP_TRY(lr.h_desc->reserve(total_bytes));
-    HIP_TRY(lr.desc->reserve(total_bytes));
+    HIP_TRY(grow_beside(c, *lr.h_desc, total_bytes));
+    HIP_TRY(grow_beside(c, *lr.desc, total_bytes));
     std::memcpy(lr.h_desc->p, hd.p, desc_bytes);
     order_items(db, sh, &hd[0], &hd[nt], nt, (uint2*)((char*)lr.h_desc->p + desc_bytes));
     *lr.desc_used = total_bytes;
@@ -833,7 +929,7 @@ int enqueue_tasks(swh_ctx* c, swh_db* db, const Task* t, int nt, const Shape& sh
     if (lr.border->fixed && border_need > lr.border->bytes)
         return fail(SWH_ERR_CAPACITY, "strip borders need %llu bytes for one workgroup, above "
                     "the fixed %zu", (unsigned long long)border_need, lr.border->bytes);
-    HIP_TRY(lr.border->reserve(border_need));
+    HIP_TRY(grow_beside(c, *lr.border, border_need, true));   // within the border budget
 
     ScanArgs a{};
     a.words = (const uint32_t*)db->words.p;
@@ -936,15 +1032,16 @@ int run_queries(swh_ctx* c, swh_db* db_in, const uint8_t* const* qs, const int* 
     }
     const size_t nseq = db->nseq;
     const size_t score_bytes = (size_t)nq * nseq * sizeof(int32_t);
-    HIP_TRY(c->h_prof.reserve_grow(prof_elems * sizeof(int16_t)));
-    HIP_TRY(c->prof.reserve_grow(prof_elems * sizeof(int16_t)));
-    HIP_TRY(c->scores.reserve_grow(score_bytes));
-    HIP_TRY(c->ovf.reserve_grow((size_t)nq * nseq * sizeof(uint32_t)));
-    HIP_TRY(c->counters.reserve((16 + (size_t)nq) * sizeof(uint32_t)));
-    HIP_TRY(c->h_ctr.reserve((16 + (size_t)nq) * sizeof(uint32_t)));
+    // (a database build may be running beside this search: no buffer is freed)
+    HIP_TRY(grow_beside(c, c->h_prof, prof_elems * sizeof(int16_t)));
+    HIP_TRY(grow_beside(c, c->prof, prof_elems * sizeof(int16_t)));
+    HIP_TRY(grow_beside(c, c->scores, score_bytes));
+    HIP_TRY(grow_beside(c, c->ovf, (size_t)nq * nseq * sizeof(uint32_t)));
+    HIP_TRY(grow_beside(c, c->counters, (16 + (size_t)nq) * sizeof(uint32_t)));
+    HIP_TRY(grow_beside(c, c->h_ctr, (16 + (size_t)nq) * sizeof(uint32_t)));
     const bool direct_out = is_pinned_host(scores_host);
     const bool pinned_out = !direct_out && score_bytes <= (size_t(64) << 20);
-    if (pinned_out) HIP_TRY(c->h_scores.reserve_grow(score_bytes));
+    if (pinned_out) HIP_TRY(grow_beside(c, c->h_scores, score_bytes));
     int16_t* hp = (int16_t*)c->h_prof.p;
     std::vector<Task> tasks(nq);
     for (int i = 0; i < nq; ++i) {
@@ -960,16 +1057,19 @@ int run_queries(swh_ctx* c, swh_db* db_in, const uint8_t* const* qs, const int* 
         T.lane_end = db->n_blocks;
     }
     uint32_t* ctr = (uint32_t*)c->counters.p;
-    // the launch sequence after the profile upload: scan, rescore, copy-back
-    auto enqueue_rest = [&](swh_stats* pst) -> int {
-        int r2 = enqueue_tasks(c, db, tasks.data(), nq, sh, o, ctr, ctx_res(c), pst);
-        if (r2) return r2;
+    // the launch sequence after the profile upload: scan and rescore, then
+    // the copy-back
+    auto enqueue_copyback = [&]() -> int {
         HIP_TRY(hipMemcpyAsync(c->h_ctr.p, ctr, (16 + (size_t)nq) * sizeof(uint32_t),
                                hipMemcpyDeviceToHost, c->stream));
         if (pinned_out || direct_out)
             HIP_TRY(hipMemcpyAsync(direct_out ? (void*)scores_host : c->h_scores.p, c->scores.p,
                                    score_bytes, hipMemcpyDeviceToHost, c->stream));
         return SWH_OK;
+    };
+    auto enqueue_rest = [&](swh_stats* pst) -> int {
+        int r2 = enqueue_tasks(c, db, tasks.data(), nq, sh, o, ctr, ctx_res(c), pst);
+        return r2 ? r2 : enqueue_copyback();
     };
     std::vector<long long> key;
     if (o.use_graph) {
@@ -999,9 +1099,18 @@ int run_queries(swh_ctx* c, swh_db* db_in, const uint8_t* const* qs, const int* 
         HIP_TRY(hipMemcpyAsync(c->prof.p, hp, prof_elems * sizeof(int16_t),
                                hipMemcpyHostToDevice, c->stream));
         HIP_TRY(hipEventRecord(c->ev[0], c->stream));
-        rc = enqueue_rest(&st);
+        rc = enqueue_tasks(c, db, tasks.data(), nq, sh, o, ctr, ctx_res(c), &st);
         if (rc) return rc;
         HIP_TRY(hipEventRecord(c->ev[1], c->stream));
+        // The copy-back is queued once the launches have ended, not behind
+        // them: a copy waiting for a running launch holds up the copy engine,
+        // and with it the uploads of a database that another thread builds
+        // meanwhile (the worker pipeline builds chunk k + 1 while chunk k is
+        // searched; measured on MI355X, its uploads then ran only after the
+        // scan, and with this wait beside it).
+        HIP_TRY(hipEventSynchronize(c->ev[1]));
+        rc = enqueue_copyback();
+        if (rc) return rc;
     }
     HIP_TRY(hipStreamSynchronize(c->stream));
     if (o.use_graph && !replay) {

@@ -251,12 +251,24 @@ struct swh_ctx {
     BufCache cache;                     // buffers of destroyed databases
     std::mutex stage_mu;                // one database upload at a time
     DevBuf stage_raw, stage_soff;       // its staging buffers, reused
+    PinnedBuf h_stage;                  // ... and the page-locked staging of its tables
+    // Buffers that a build or a search outgrew (grow_beside).  Freeing one
+    // waits for the whole device (hipFree and hipHostFree synchronise it),
+    // which would hold a build behind a search running on another thread, or
+    // a search behind a build: the worker pipeline overlaps the two.  They
+    // are kept until the context is destroyed, up to kRetiredMax bytes.
+    std::mutex retired_mu;
+    std::vector<DevBuf> retired_dev;
+    std::vector<PinnedBuf> retired_host;
+    size_t retired_bytes = 0;
+    static constexpr size_t kRetiredMax = size_t(4) << 30;
     std::vector<swh_tune_entry> tune;   // empty: built-in shape choice
     uint64_t tune_gen = 0;              // bumped whenever the table changes
     // A server's loop thread launches on this context's stream with its
     // descriptor, border and counter buffers; while one is attached every
     // other call that would use them is refused (SWH_ERR_STATE).
     std::atomic<swh_server*> server{nullptr};
+    std::atomic<int> live_dbs{0};       // databases (layouts) made on it and not destroyed
     // The allocation-free search (swh_search_noalloc): this device's static
     // scratch array and a page-locked descriptor staging area of fixed size,
     // both set up by swh_ctx_create.
@@ -271,6 +283,7 @@ struct swh_db {
     std::vector<uint64_t> blk_real;     // prefix sums over blocks of real residues
     std::vector<uint64_t> blk_padded;   // ... and of padded residues (rows x sequences)
     uint64_t nseq = 0, residues = 0, padded = 0;
+    uint64_t dev_bytes = 0;     // device bytes the layout uses (its buffers may be larger)
     int n_pairs = 0;            // group-mode pair blocks, blocks [0, n_pairs)
     int n_blocks = 0;           // all blocks; lane blocks are [n_pairs, n_blocks)
     uint32_t max_rows_lane = 0, max_rows_pair = 0;
@@ -344,6 +357,13 @@ inline LaunchRes ctx_res(swh_ctx* c) {
 }
 // SWH_ERR_STATE (with a message naming what) while a server is attached to c.
 int refuse_if_served(const swh_ctx* c, const char* what);
+// Grows b to at least want (1.5 times its old size when that is more, unless
+// exact; a fixed buffer cannot grow) and keeps its old allocation in c's
+// retired list instead of freeing it now.
+hipError_t grow_beside(swh_ctx* c, DevBuf& b, size_t want, bool exact = false);
+hipError_t grow_beside(swh_ctx* c, PinnedBuf& b, size_t want, bool exact = false);
+// Frees c's retired buffers (waits for the device).
+void free_retired(swh_ctx* c);
 
 int enqueue_tasks(swh_ctx* c, swh_db* db, const Task* t, int nt, const Shape& sh,
                   const swh_search_opts& o, uint32_t* counters, const LaunchRes& lr,

@@ -10,6 +10,7 @@ All work happens in libswh.so; this module only marshals arrays.
 from __future__ import annotations
 
 import ctypes as C
+import weakref
 
 import numpy as np
 
@@ -78,6 +79,7 @@ class Engine:
         N.check(N.lib().swh_ctx_create(int(device), N.ptr(self.matrix), self.gap_open,
                                        self.gap_extend, C.byref(h)))
         self.handle = h
+        self._dbs = weakref.WeakSet()   # open databases, closed before the context
 
     def set_scoring(self, matrix, gap_open, gap_extend):
         m = _as_matrix(matrix)
@@ -100,7 +102,9 @@ class Engine:
                         int(queries_per_launch))
         N.check(N.lib().swh_db_create(self.handle, N.ptr(res), N.ptr(off), N.ptr(lens),
                                       db.n, C.byref(o), C.byref(h)))
-        return Database(self, h, db.n, db.total)
+        d = Database(self, h, db.n, db.total)
+        self._dbs.add(d)
+        return d
 
     def pinned_scores(self, n, nq=1):
         """A page-locked int32 array of nq x n (1-d for nq = 1) for the out=
@@ -176,7 +180,11 @@ class Engine:
         return [arr[i].as_dict() for i in range(n.value)]
 
     def close(self):
+        """Closes the databases this engine loaded, then the context (the C
+        ABI refuses to destroy a context whose databases still exist)."""
         if self.handle:
+            for d in list(self._dbs):
+                d.close()
             N.check(N.lib().swh_ctx_destroy(self.handle))
             self.handle = None
 

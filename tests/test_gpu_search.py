@@ -382,6 +382,61 @@ def test_tuning_table_validation():
     e.close()
 
 
+def test_context_destroy_refused_while_its_databases_exist():
+    """swh_ctx_destroy refuses while a database made on the context exists
+    (its buffers go back to the context when it is destroyed); Engine.close
+    closes its databases first."""
+    from swhybrid_amd import _native as N
+    e = Engine("blosum62", 10, 2)
+    db = small_db(300, 97)
+    d = e.load_db(db)
+    d2 = e.load_db(db)
+    assert N.lib().swh_ctx_destroy(e.handle) == -4            # SWH_ERR_STATE
+    assert b"still exist" in N.lib().swh_last_error()
+    q = G.queries([120], 98)[0]
+    assert np.array_equal(e.search(d, q)[0], O.search_i32(q, db, e.matrix, 10, 2))
+    d2.close()
+    e.close()
+    assert d.handle is None and e.handle is None
+
+
+def test_build_beside_a_running_search_and_pageable_uploads(eng):
+    """A database built on one thread while a search runs on another (the
+    worker pipeline's overlap) scores like the oracle; so do databases whose
+    residues come from pageable memory, small (through the page-locked
+    staging) and above 64 MB (a prefix the runtime stages and a padded tail),
+    and one from page-locked memory."""
+    import threading
+    from swhybrid_amd import _native as N
+    big = small_db(120_000, 99, mean=600, lo=20, hi=3000)        # ~72M residues
+    assert big.total > (64 << 20)
+    q = G.queries([2000], 100)[0]
+    d_big = eng.load_db(big)
+    got = {}
+
+    def search():
+        got["big"] = eng.search(d_big, q)[0]
+
+    small = small_db(5000, 101)
+    t = threading.Thread(target=search)
+    t.start()
+    d_small = eng.load_db(small)                                  # beside the search
+    t.join()
+    assert np.array_equal(eng.search(d_small, q)[0],
+                          O.search_i32(q, small, eng.matrix, eng.gap_open, eng.gap_extend))
+    idx = np.random.Generator(np.random.PCG64(102)).choice(big.n, 400, replace=False)
+    want = O.search_i32(q, big.subset(idx), eng.matrix, eng.gap_open, eng.gap_extend)
+    assert np.array_equal(got["big"][idx], want)
+    # page-locked residues go up as they are
+    pin = N.pinned_array(small.total, np.uint8)
+    pin[:] = small.residues
+    pinned = G.SeqDB(pin, small.offsets, small.lengths)
+    d_pin = eng.load_db(pinned)
+    assert np.array_equal(eng.search(d_pin, q)[0], eng.search(d_small, q)[0])
+    for x in (d_big, d_small, d_pin):
+        x.close()
+
+
 def test_small_database_second_layout(eng):
     """A database too small to fill the device with lane items of one query
     carries a second layout with every sequence of 128+ residues in lane
