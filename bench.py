@@ -358,7 +358,7 @@ def run_dbfile(eng):
             "top10_equals_resident_search": same, "titles_ok": titles_ok}
 
 
-def pipeline_breakdown(db, qlen, chunk_bytes=0, runs=2):
+def pipeline_breakdown(db, qlen, chunk_bytes=0, runs=2, first_chunk_bytes=0):
     """The worker pipeline (DataPool -> HIPWorker, no CPU worker) over the
     5M-sequence database in host memory for one query length: where the
     HIPWorker's time goes (first fill, aux fills and their swh_db_create
@@ -368,11 +368,12 @@ def pipeline_breakdown(db, qlen, chunk_bytes=0, runs=2):
     from swhybrid_amd.engine import hybrid_search
     q = G.queries([qlen], 5000 + qlen)[0]
     for _ in range(runs):
-        r = hybrid_search(db, q, k=TOP_K, hip_workers=1, cpu_threads=0, chunk_bytes=chunk_bytes)
+        r = hybrid_search(db, q, k=TOP_K, hip_workers=1, cpu_threads=0, chunk_bytes=chunk_bytes,
+                          first_chunk_bytes=first_chunk_bytes)
     rep = r["report"]
     cells = db.total * qlen
-    default = (1 << 30) if qlen >= 2700 else (512 << 20)   # the pool's default chunk
-    out = {"qlen": qlen, "chunk_bytes": chunk_bytes or default,
+    out = {"qlen": qlen, "chunk_bytes": rep["chunk_bytes"],
+           "first_chunk_bytes": rep["first_chunk_bytes"],
            "gcups_reference_measure": round(rep["gcups"], 1),
            "gcups_wall": round(cells / rep["wall_seconds"] / 1e9, 1),
            "seconds": round(rep["seconds"], 4), "wall_seconds": round(rep["wall_seconds"], 4)}
@@ -388,8 +389,9 @@ def pipeline_breakdown(db, qlen, chunk_bytes=0, runs=2):
 def run_hybrid(db, qlens=(1000, 5478)):
     """The reference's hybrid scheduling: the worker pipeline over the same
     5M-sequence database held in host memory, chunks of the pool's default
-    size (512 MB below query length 2,700, 1 GB from there) to the
-    HIPWorker and of 4 MB to the CPU worker (the host's AVX2 cores), all
+    schedule (512 MB below query length 600, 256 MB below 2,500, from there a
+    256 MB first chunk, then 1 GB) to the HIPWorker and of 4 MB to the CPU
+    worker (the host's AVX2 cores), all
     from one DataPool; HIPWorker alone first, then both.  `*_gcups` is the
     reference's own measure (first worker start to last stop, after the
     first chunk is filled); `*_wall_gcups` times the whole call."""
@@ -544,6 +546,8 @@ def main():
                     help="only run the worker pipeline at this query length and print where "
                          "its time goes")
     ap.add_argument("--pipeline-chunk-mb", type=int, default=0)
+    ap.add_argument("--pipeline-first-mb", type=int, default=0,
+                    help="first chunk of the schedule (later chunks double up to the chunk size)")
     ap.add_argument("--cpu-sweep", action="store_true",
                     help="only measure the host scan's query segmentation and print it")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -573,7 +577,8 @@ def main():
     t0 = time.perf_counter()
     db = G.c2_db(n=args.nseq)
     if args.pipeline:
-        print(json.dumps(pipeline_breakdown(db, args.pipeline, args.pipeline_chunk_mb << 20)))
+        print(json.dumps(pipeline_breakdown(db, args.pipeline, args.pipeline_chunk_mb << 20,
+                                            first_chunk_bytes=args.pipeline_first_mb << 20)))
         return
     queries = G.c2_queries()
     if args.query_lengths:

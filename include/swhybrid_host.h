@@ -21,9 +21,9 @@ extern "C" {
 const char* swhy_last_error(void);
 
 /* Search one query through the worker pipeline.
- *   chunk_bytes    DataPool chunk budget per fill (0 = 512 MB for queries
- *                  shorter than 2,700 residues, 1 GB from there; the
- *                  reference's CUDAWorker uses 256 MB, src/CUDAWorker.cu:44)
+ *   chunk_bytes    DataPool chunk budget per fill (0 = the default schedule
+ *                  of swhy_pool_opts.chunk_bytes; the reference's CUDAWorker
+ *                  uses 256 MB, src/CUDAWorker.cu:44)
  *   n_workers      HIPWorkers on devices 0..n_workers-1
  *   out_top_*      topk entries: score, caller index, pool index; ties are
  *                  ordered by pool index (the order sequences are handed out),
@@ -65,10 +65,14 @@ typedef struct swhy_pool_opts {
     int topk;
     int hip_workers;           /* HIPWorkers on devices 0..hip_workers-1 */
     int cpu_threads;           /* threads of the CPU worker; 0 = no CPU worker */
-    uint64_t chunk_bytes;      /* HIPWorker chunk (0 = 512 MB below query length
-                                  2,700, 1 GB from there) */
+    uint64_t chunk_bytes;      /* HIPWorker chunk (0 = the default schedule: 512 MB
+                                  below query length 600, 256 MB below 2,500, from
+                                  there a 256 MB first chunk, then 1 GB) */
     uint64_t cpu_chunk_bytes;  /* CPU worker chunk (0 = 4 MB); the CPU worker takes no
                                   chunk once only hip_workers x chunk_bytes are left */
+    uint64_t first_chunk_bytes; /* a HIPWorker's first chunk; each later one doubles, up
+                                  to chunk_bytes (0 = chunk_bytes when that is set,
+                                  else the default schedule) */
 } swhy_pool_opts;
 
 typedef struct swhy_pool_report {
@@ -88,6 +92,8 @@ typedef struct swhy_pool_report {
     double dev_aux_wait_s;     /* waits for the aux thread after a search */
     double dev_kernel_s;       /* scan + rescore kernels of the searches */
     int dev_chunks;            /* chunks searched */
+    uint64_t chunk_bytes;      /* the HIPWorkers' chunk schedule that ran: */
+    uint64_t first_chunk_bytes; /*   first chunk, doubling up to chunk_bytes */
 } swhy_pool_report;
 
 /* swhy_pool_search with a CPU worker beside the HIPWorkers.  out_top_* get

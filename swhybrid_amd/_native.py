@@ -185,7 +185,8 @@ HOST_EXPORTS = ("swhy_last_error", "swhy_pool_search", "swhy_dbfile_make", "swhy
 
 class SwhyPoolOpts(C.Structure):
     _fields_ = [("topk", C.c_int), ("hip_workers", C.c_int), ("cpu_threads", C.c_int),
-                ("chunk_bytes", C.c_uint64), ("cpu_chunk_bytes", C.c_uint64)]
+                ("chunk_bytes", C.c_uint64), ("cpu_chunk_bytes", C.c_uint64),
+                ("first_chunk_bytes", C.c_uint64)]
 
 
 class SwhyPoolReport(C.Structure):
@@ -195,7 +196,8 @@ class SwhyPoolReport(C.Structure):
                 ("wall_seconds", C.c_double), ("dev_first_fill_s", C.c_double),
                 ("dev_aux_fill_s", C.c_double), ("dev_pack_s", C.c_double),
                 ("dev_search_s", C.c_double), ("dev_aux_wait_s", C.c_double),
-                ("dev_kernel_s", C.c_double), ("dev_chunks", C.c_int)]
+                ("dev_kernel_s", C.c_double), ("dev_chunks", C.c_int),
+                ("chunk_bytes", C.c_uint64), ("first_chunk_bytes", C.c_uint64)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

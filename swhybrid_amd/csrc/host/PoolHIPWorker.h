@@ -9,14 +9,20 @@
 //   search                  also writes every score into the pool's sink
 //                           (pool order), for parity checks
 //   busySeconds             device time of the worker's searches
+//   fillBytes               a chunk schedule: the first chunk firstChunkBytes,
+//                           each later one twice the one before, up to
+//                           chunkBytes (0: every chunk chunkBytes)
 #pragma once
+#include <algorithm>
+
 #include "HIPWorker.h"
 
 class PoolHIPWorker : public HIPWorker {
   public:
     PoolHIPWorker(DataPool* loader, Param* params, RecalcWorker* recalcWorker, int deviceId,
-                  size_t chunkBytes)
-        : HIPWorker(loader, params, recalcWorker, deviceId, chunkBytes) {}
+                  size_t chunkBytes, size_t firstChunkBytes = 0)
+        : HIPWorker(loader, params, recalcWorker, deviceId, chunkBytes),
+          firstChunkBytes_(firstChunkBytes) {}
     ~PoolHIPWorker() override {
         freeChunk(dbSeqBuf);
         dbSeqBuf = nullptr;
@@ -40,4 +46,14 @@ class PoolHIPWorker : public HIPWorker {
                 sink_[compEntry->getBaseIdx() + i] = s[i];
         }
     }
+
+    size_t fillBytes(int k) const override {
+        if (firstChunkBytes_ == 0 || firstChunkBytes_ >= bufferSize) return bufferSize;
+        size_t b = firstChunkBytes_;
+        for (int i = 0; i < k && b < bufferSize; ++i) b *= 2;
+        return std::min(b, bufferSize);
+    }
+
+  private:
+    size_t firstChunkBytes_;
 };

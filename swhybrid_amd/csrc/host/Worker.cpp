@@ -1,5 +1,6 @@
 #include "Worker.h"
 
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -41,7 +42,9 @@ bool Worker::fill() {
         info = new batchInfo[infoCapacity];
         if (!dbSeqBuf) throw "worker: chunk buffer allocation failed\n";
     }
-    bool ret = loader_->fillChunk(dbSeqBuf, info, infoSize, filledSize, bufferSize, cpuFlag,
+    const size_t cap = std::max(std::min(fillBytes(fills_++), bufferSize),
+                                loader_->maxBatchBytes());
+    bool ret = loader_->fillChunk(dbSeqBuf, info, infoSize, filledSize, cap, cpuFlag,
                                   fillBaseIndex);
     if (infoSize != 0) packDB();
     return ret;

@@ -94,6 +94,12 @@ class Worker {
     // overrides these frees dbSeqBuf in its own destructor
     virtual char* allocChunk(size_t bytes);
     virtual void freeChunk(char* p);
+    // bytes the DataPool may put into fill k of this worker (k = 0 first;
+    // at most bufferSize, at least one batch)
+    virtual size_t fillBytes(int k) const {
+        (void)k;
+        return bufferSize;
+    }
     bool fill();
     void auxWork();
     void swapEntries() {
@@ -127,6 +133,7 @@ class Worker {
     size_t filledSize = 0;
     size_t fillBaseIndex = 0;
     size_t bufferSize = 0;
+    int fills_ = 0;   // fills so far
     size_t TotalAminoAcidResidue = 0;
     bool cpuFlag = false;
     int score_limit_7 = 0;   // 128 - max(matrix) (src/Worker.cpp:62-70)

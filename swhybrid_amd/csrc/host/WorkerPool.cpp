@@ -3,12 +3,14 @@
 #include <algorithm>
 #include <thread>
 
-WorkerPool::WorkerPool(DataPool* loader, Param* params, size_t chunkBytes, size_t cpuChunkBytes)
+WorkerPool::WorkerPool(DataPool* loader, Param* params, size_t chunkBytes, size_t cpuChunkBytes,
+                       size_t firstChunkBytes)
     : loader_(loader), params_(params), resultList_(params->topNum) {
     recalcWorker_ = new RecalcWorker(params, 0);
     recalcWorker_->setQuery(loader->getQuery(), loader->getQueryLen());
     for (int i = 0; i < params->hipNum; ++i) {
-        workerList_.push_back(new PoolHIPWorker(loader, params, recalcWorker_, i, chunkBytes));
+        workerList_.push_back(new PoolHIPWorker(loader, params, recalcWorker_, i, chunkBytes,
+                                                firstChunkBytes));
         isCpu_.push_back(false);
     }
     if (params->cpuNum > 0) {

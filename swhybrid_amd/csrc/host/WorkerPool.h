@@ -25,7 +25,10 @@ class WorkerPool {
     // params->cpuNum > 0, one CPUWorker with that many threads (chunks of
     // cpuChunkBytes), in that order (the reference creates the CPU worker
     // first, src/WorkerPool.cpp:47-52; the order only decides ties).
-    WorkerPool(DataPool* loader, Param* params, size_t chunkBytes, size_t cpuChunkBytes = 0);
+    // chunkBytes: each HIPWorker's chunk; firstChunkBytes (0 = chunkBytes):
+    // its first chunk, each later one doubling up to chunkBytes
+    WorkerPool(DataPool* loader, Param* params, size_t chunkBytes, size_t cpuChunkBytes = 0,
+               size_t firstChunkBytes = 0);
     ~WorkerPool();
     // Start every worker, wait for them, rescore, merge.  0 on success.
     int run(int* scoreSink);

@@ -21,18 +21,36 @@
 
 namespace {
 thread_local std::string g_err;
-// HIPWorker chunk by default: 512 MB (twice the reference's 256 MB,
-// src/CUDAWorker.cu:44) for queries shorter than 2,700 residues, 1 GB from
-// there.  A chunk is one launch, and larger launches lose less to their tails
-// and to the uploads of the next chunk (5M sequences in host memory, query
-// length 1000 / 5478: kernels at 8,253 / 8,464 GCUPS with 512 MB chunks and
-// 9,089 / 9,132 with 1 GB; first worker start to last stop 7,040 / 8,220
-// against 7,916 / 8,907 GCUPS).  But the first chunk's fill, upload and packing
-// overlap nothing (48 ms at 512 MB, 98 ms at 1 GB), so for the whole call
-// 512 MB wins at 1000 (5,712 against 5,314 GCUPS) and 1 GB at 5478 (8,050
-// against 7,796); the two cross near query length 2,700.
-size_t default_chunk(int qlen) {
-    return qlen >= 2700 ? size_t(1) << 30 : size_t(512) << 20;
+// A HIPWorker's chunk schedule: its first chunk, then each one twice the one
+// before, up to chunk.
+struct ChunkPlan {
+    size_t chunk, first;
+};
+// The default schedule, from the whole call (first chunk's fill and build
+// included, which nothing overlaps) over the 5M-sequence database in host
+// memory on MI355X.  A chunk is one launch: larger ones lose less to launch
+// tails, and the reference's measure (first worker start to last stop) is
+// best with 1 GB chunks at every query length.  But the first chunk's fill and
+// build take 0.1 ms per MB before any search, and below query length 600 the
+// host's fills (about 0.09 ms per MB with the build) outlast the searches.
+// Whole call, GCUPS (256 MB / 512 MB / 1 GB fixed, 256 MB then 1 GB):
+//   query 144:  1,218 / 1,273 / 1,039        464:  3,871 / 3,950 / 3,124
+//   query 1000: 6,102-6,654 / 5,923-6,356 / 5,263-5,420 / 4,929
+//   query 3000: 7,718-7,858 / 7,530 / 7,482-7,534 / 8,122
+//   query 5478: 8,032-8,041 / 8,015-8,038 / 8,175-8,192 / 8,253
+// A small first chunk holds only the longest sequences (the pool hands them
+// out first), whose launch balances poorly, so it pays only where later
+// chunks run long enough (from query length 2,500).
+ChunkPlan default_chunks(int qlen) {
+    constexpr size_t MB = size_t(1) << 20;
+    if (qlen < 600) return ChunkPlan{512 * MB, 512 * MB};
+    if (qlen < 2500) return ChunkPlan{256 * MB, 256 * MB};
+    return ChunkPlan{1024 * MB, 256 * MB};
+}
+ChunkPlan chunk_plan(const swhy_pool_opts& o, int qlen) {
+    ChunkPlan p = o.chunk_bytes ? ChunkPlan{o.chunk_bytes, o.chunk_bytes} : default_chunks(qlen);
+    if (o.first_chunk_bytes) p.first = std::min<size_t>(o.first_chunk_bytes, p.chunk);
+    return p;
 }
 }
 
@@ -51,8 +69,10 @@ int run_pool(DataPool& pool, uint64_t nseq, const int32_t* matrix, int gap_open,
     swhy_pool_report r{};
     const auto t0 = std::chrono::steady_clock::now();
     {
-        WorkerPool wp(&pool, &params, o.chunk_bytes ? o.chunk_bytes : default_chunk((int)pool.getQueryLen()),
-                      o.cpu_chunk_bytes);
+        const ChunkPlan plan = chunk_plan(o, (int)pool.getQueryLen());
+        WorkerPool wp(&pool, &params, plan.chunk, o.cpu_chunk_bytes, plan.first);
+        r.chunk_bytes = plan.chunk;
+        r.first_chunk_bytes = plan.first;
         const int rc = wp.run(out_scores ? sink.data() : nullptr);
         if (rc != 0) {
             g_err = "worker pipeline: " + wp.error();
@@ -165,7 +185,7 @@ int swhy_pool_search(const uint8_t* residues, const uint64_t* offsets, const uin
     }
     try {
         DataPool pool(residues, offsets, lengths, nseq, query, (size_t)qlen);
-        const swhy_pool_opts o{topk, n_workers, 0, chunk_bytes, 0};
+        const swhy_pool_opts o{topk, n_workers, 0, chunk_bytes, 0, 0};
         swhy_pool_report r{};
         const int rc = run_pool(pool, nseq, matrix, gap_open, gap_extend, o, out_top_score,
                                 out_top_index, out_top_pool_index, out_scores, &r);
@@ -272,7 +292,7 @@ int swhy_dbfile_pool_search(swhy_dbfile* h, const uint8_t* query, int qlen,
     }
     try {
         DataPool pool(*h->f, query, (size_t)qlen);
-        const swhy_pool_opts o{topk, n_workers, 0, chunk_bytes, 0};
+        const swhy_pool_opts o{topk, n_workers, 0, chunk_bytes, 0, 0};
         swhy_pool_report r{};
         const int rc = run_pool(pool, h->f->slots(), matrix, gap_open, gap_extend, o,
                                 out_top_score, out_top_index, nullptr, out_scores, &r);

@@ -109,7 +109,7 @@ int main(int argc, char** argv) {
     std::vector<int64_t> ti(a.top);
     if (nq == 1 && !a.batch) {
         std::printf("Query sequence length %d\n", qlens[0]);
-        const swhy_pool_opts o{a.top, a.gpus, a.cpus, 0, 0};
+        const swhy_pool_opts o{a.top, a.gpus, a.cpus, 0, 0, 0};
         swhy_pool_report rep{};
         if (swhy_dbfile_pool_search_ex(db, qs[0], qlens[0], matrix, a.open, a.extend, &o,
                                        ts.data(), ti.data(), nullptr, &rep) != 0)

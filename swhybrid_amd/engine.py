@@ -331,7 +331,7 @@ def cpu_search(db: SeqDB, query, matrix="blosum62", gap_open=10, gap_extend=2, t
 
 def hybrid_search(db: SeqDB, query, matrix="blosum62", gap_open=10, gap_extend=2, k=10,
                   hip_workers=1, cpu_threads=0, chunk_bytes=0, cpu_chunk_bytes=0,
-                  all_scores=False):
+                  all_scores=False, first_chunk_bytes=0):
     """The worker pipeline with HIPWorkers and a CPU worker (AVX2, cpu_threads
     threads) pulling chunks from one DataPool.  Returns dict(top_score,
     top_index, scores, report)."""
@@ -344,7 +344,7 @@ def hybrid_search(db: SeqDB, query, matrix="blosum62", gap_open=10, gap_extend=2
     ti = np.zeros(k, dtype=np.int64)
     sc = np.zeros(db.n, dtype=np.int32) if all_scores else None
     o = N.SwhyPoolOpts(int(k), int(hip_workers), int(cpu_threads), int(chunk_bytes),
-                       int(cpu_chunk_bytes))
+                       int(cpu_chunk_bytes), int(first_chunk_bytes))
     rep = N.SwhyPoolReport()
     L = N.host_lib()
     rc = L.swhy_pool_search_ex(N.ptr(res), N.ptr(off), N.ptr(lens), db.n, N.ptr(q), len(q),

@@ -49,6 +49,26 @@ def pool_order(db):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("chunk_bytes,first", [(40_000, 5_000), (300_000, 20_000)])
+def test_chunk_schedule_matches_oracle(chunk_bytes, first):
+    """A chunk schedule (a small first chunk, each later one twice the one
+    before up to chunk_bytes) scores every sequence like the oracle and runs
+    the schedule it reports."""
+    from swhybrid_amd.engine import hybrid_search
+    rng = np.random.Generator(np.random.PCG64(73))
+    q = G.queries([250], 74)[0]
+    db = G.SeqDB.from_lengths(rng, G.lognormal_lengths(rng, 2500, 250, 0.7, 5, 3000))
+    r = hybrid_search(db, q, k=10, hip_workers=1, cpu_threads=0, chunk_bytes=chunk_bytes,
+                      first_chunk_bytes=first, all_scores=True)
+    assert np.array_equal(r["scores"], O.search_i32(q, db, A.BLOSUM62))
+    rep = r["report"]
+    assert (rep["chunk_bytes"], rep["first_chunk_bytes"]) == (chunk_bytes, first)
+    # doubling from `first`: fewer chunks than fixed `first`-sized ones, more
+    # than fixed chunk_bytes ones
+    assert db.total / first > rep["dev_chunks"] > db.total / chunk_bytes
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("chunk_bytes", [0, 40_000, 7_000])
 def test_pool_search_matches_oracle(chunk_bytes):
     rng = np.random.Generator(np.random.PCG64(71))
