@@ -692,13 +692,66 @@ static double builtin_cost(int k) {
     }
 }
 
+// Group width by a model of the launch: the larger of its throughput time
+// and its longest group pair's time.  A pair is swept by one wave, P passes of
+// rows + G - 1 steps; a step of W columns costs about 105, 159 and 238 VALU
+// instructions with wait states at W = 8, 16, 24, at about 6 cycles an
+// instruction for a wave that shares its SIMD.  Throughput: the launch's
+// cells at 9.4 T cells/s (per 256 CUs), the group share of them divided by
+// the lanes' use (group_lanes_for), times what a group-heavy launch reaches of
+// that at each width (1.45, 1.55, 1.85 at W = 24, 16, 8).  Another width
+// replaces W when its time is 5% below W's.  Measured on 100k sequences of up
+// to 2,000 residues, 90% of the cells in group pairs (ms at W = 8 / 16 / 24):
+// query 50 0.53 / 0.81 / 1.05, 100 0.74 / 0.81 / 1.07, 144 0.95 / 0.82 /
+// 1.11, 200 1.42 / 1.19 / 1.15, 256 1.47 / 1.22 / 1.37, 464 3.16 / 2.47 /
+// 2.25.  On the 5M-sequence database (7% of the cells in pairs) the longest
+// pair takes 21 ms against 27 ms of throughput at query length 144 and the
+// widths differ by less than 5%: W stays.
+static int group_width_by_latency(const swh_ctx* c, const int* qlens, int nq, const swh_db* db,
+                                  int W) {
+    if (db->n_pairs == 0 || db->max_rows_pair == 0 || db->residues == 0) return W;
+    uint64_t cells = 0;
+    int qmax = 0;
+    for (int i = 0; i < nq; ++i) {
+        cells += (uint64_t)qlens[i] * db->residues;
+        qmax = std::max(qmax, qlens[i]);
+    }
+    const double t_tp = (double)cells / (9.4e12 * std::max(c->num_cus, 1) / 256.0);
+    const double gf = (double)db->residues_pair / (double)db->residues;
+    const int min_lanes = group_min_lanes(c, db->max_rows_pair, c->border_budget);
+    auto predict = [&](int w) {
+        const double step = w == 8 ? 105.0 : w == 16 ? 159.0 : 238.0;
+        const double k = w == 8 ? 1.85 : w == 16 ? 1.55 : 1.45;
+        const int passes_max = group_pass_budget(c, cells, db->max_rows_pair, w);
+        const int G = group_lanes_for(qmax, w, passes_max, min_lanes);
+        const int strips = (qmax + w - 1) / w;
+        const int passes = (strips + G - 1) / G;
+        const double use =
+            (double)strips / ((double)passes * G) * (double)((kWave / G) * G) / kWave;
+        const double crit = passes * ((double)db->max_rows_pair + G - 1) * step * 6.0 / 2.4e9;
+        return std::max(k * t_tp * (gf / use + (1.0 - gf)), crit);
+    };
+    int best = W;
+    double t_best = predict(W) * 0.95;
+    for (int w : {16, 8}) {
+        if (w == W) continue;
+        const double t = predict(w);
+        if (t < t_best) {
+            best = w;
+            t_best = t;
+        }
+    }
+    return best;
+}
+
 // Launch shape shared by the queries of a launch.  Lane scan: the width and
 // profile placement with the least predicted time, summed over the queries as
 // cost per lane column x lane columns (lane_cols).  Costs come from the tuning
 // table when one is installed, else from builtin_cost.  Group scan: the
 // width of the bucket holding most of the batch's columns, else 16 or 24 by
 // lane use (24 is instantiated with 24-column lanes only; 8 columns per lane
-// lose 2% on the 5M-sequence database at query length 144).
+// lose 2% on the 5M-sequence database at query length 144), unless the
+// launch is bound by its longest pair (group_width_by_latency).
 Shape choose_shape(const swh_ctx* c, const int* qlens, int nq, const swh_search_opts& o,
                    const swh_db* db, uint64_t db_residues) {
     Shape sh{16, o.group_width, true};
@@ -750,6 +803,9 @@ Shape choose_shape(const swh_ctx* c, const int* qlens, int nq, const swh_search_
             if (c->tune[b].group_width > 0) sh.Wg = c->tune[b].group_width;   // 0: built-in
         }
         if (sh.Wg == 24 && sh.Wl != 24) sh.Wg = 16;   // a bucket tuned with other lanes
+        // a tuned width was measured on a throughput-bound launch; a launch on
+        // another database may be bound by its longest pair instead
+        sh.Wg = group_width_by_latency(c, qlens, nq, db, sh.Wg);
     }
     return sh;
 }

@@ -382,6 +382,23 @@ def test_tuning_table_validation():
     e.close()
 
 
+@pytest.mark.parametrize("qlen", [20, 50, 90])
+def test_short_query_over_long_sequences_default_shape(eng, qlen):
+    """A short query over a small database of long sequences: the default
+    launch shape (group width 8 where the longest pair bounds the launch)
+    scores like the oracle."""
+    rng = np.random.Generator(np.random.PCG64(103 + qlen))
+    db = G.SeqDB.from_lengths(rng, G.lognormal_lengths(rng, 3000, 500, 0.8, 20, 2000))
+    q = G.queries([qlen], 104 + qlen)[0]
+    d = eng.load_db(db)
+    try:
+        got, st = eng.search(d, q)
+        assert st["cells_group"] > 0
+        assert np.array_equal(got, O.search_i32(q, db, eng.matrix, eng.gap_open, eng.gap_extend))
+    finally:
+        d.close()
+
+
 def test_context_destroy_refused_while_its_databases_exist():
     """swh_ctx_destroy refuses while a database made on the context exists
     (its buffers go back to the context when it is destroyed); Engine.close
