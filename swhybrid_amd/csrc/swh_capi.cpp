@@ -692,21 +692,69 @@ static double builtin_cost(int k) {
     }
 }
 
-// Group width by a model of the launch: the larger of its throughput time
-// and its longest group pair's time.  A pair is swept by one wave, P passes of
+// Time model of a group-mode launch, shared by the group width and pass
+// choices (group_width_by_latency).  A pair is swept by one wave, P passes of
 // rows + G - 1 steps; a step of W columns costs about 105, 159 and 238 VALU
 // instructions with wait states at W = 8, 16, 24, at about 6 cycles an
 // instruction for a wave that shares its SIMD.  Throughput: the launch's
 // cells at 9.4 T cells/s (per 256 CUs), the group share of them divided by
-// the lanes' use (group_lanes_for), times what a group-heavy launch reaches of
-// that at each width (1.45, 1.55, 1.85 at W = 24, 16, 8).  Another width
-// replaces W when its time is 5% below W's.  Measured on 100k sequences of up
-// to 2,000 residues, 90% of the cells in group pairs (ms at W = 8 / 16 / 24):
-// query 50 0.53 / 0.81 / 1.05, 100 0.74 / 0.81 / 1.07, 144 0.95 / 0.82 /
-// 1.11, 200 1.42 / 1.19 / 1.15, 256 1.47 / 1.22 / 1.37, 464 3.16 / 2.47 /
-// 2.25.  On the 5M-sequence database (7% of the cells in pairs) the longest
-// pair takes 21 ms against 27 ms of throughput at query length 144 and the
-// widths differ by less than 5%: W stays.
+// the lanes' use, times what a group-heavy launch reaches of that at each
+// width (1.45, 1.55, 1.85 at W = 24, 16, 8).
+struct GroupLaunchModel {
+    double t_tp = 0;   // the launch's cells at the scan's full rate, seconds
+    double gf = 0;     // share of the cells in group pairs
+    uint32_t rows_max = 0;
+    static double step(int w) { return w == 8 ? 105.0 : w == 16 ? 159.0 : 238.0; }
+    static double k(int w) { return w == 8 ? 1.85 : w == 16 ? 1.55 : 1.45; }
+    static double use(int qlen, int w, int G) {
+        const int strips = (qlen + w - 1) / w;
+        const int passes = (strips + G - 1) / G;
+        return (double)strips / ((double)passes * G) * (double)((kWave / G) * G) / kWave;
+    }
+    double pass_s(int w, int G) const {
+        return ((double)rows_max + G - 1) * step(w) * 6.0 / 2.4e9;
+    }
+    double throughput_s(int w, double lane_use) const {
+        return k(w) * t_tp * (gf / lane_use + (1.0 - gf));
+    }
+};
+
+static GroupLaunchModel group_model(const swh_ctx* c, const swh_db* db, uint64_t launch_cells) {
+    GroupLaunchModel m;
+    m.t_tp = (double)launch_cells / (9.4e12 * std::max(c->num_cus, 1) / 256.0);
+    m.gf = db->residues ? (double)db->residues_pair / (double)db->residues : 0.0;
+    m.rows_max = db->max_rows_pair;
+    return m;
+}
+
+// Passes a query's group items may make: the launch's pass budget
+// (group_pass_budget), unless the best group within it leaves more than a
+// tenth of the wave's lanes idle; then as many as keep the longest pair's
+// sweep within the launch's predicted time.  On 100k sequences of up to 2,000
+// residues the budget allows one pass: 42 and 64 lanes a group at query
+// length 1000 and 2005 (7.0 and 14.1 ms), against 21 lanes in 2 and 4 passes
+// (4.4 and 8.9 ms).  On the 5M-sequence and the overflow-stress databases the
+// budget's groups use 94% of the lanes or more.
+int group_passes(const swh_ctx* c, const swh_db* db, uint64_t launch_cells, int qlen, int W,
+                 int max_passes, int min_lanes) {
+    const int G = group_lanes_for(qlen, W, max_passes, min_lanes);
+    const double u = GroupLaunchModel::use(qlen, W, G);
+    if (u >= 0.9 || db->max_rows_pair == 0) return max_passes;
+    const GroupLaunchModel m = group_model(c, db, launch_cells);
+    const double t_launch = m.throughput_s(W, 0.95);
+    const int p = (int)(t_launch / m.pass_s(W, kWave));
+    return std::max(max_passes, std::min(kWave, p));
+}
+
+// Group width by the launch model (GroupLaunchModel): the larger of the
+// throughput time and the longest pair's sweep.  Another width replaces W
+// when its time is 5% below W's.  Measured on 100k sequences of up to 2,000
+// residues, 90% of the cells in group pairs (ms at W = 8 / 16 / 24): query 50
+// 0.53 / 0.81 / 1.05, 100 0.74 / 0.81 / 1.07, 144 0.95 / 0.82 / 1.11, 200
+// 1.42 / 1.19 / 1.15, 256 1.47 / 1.22 / 1.37, 464 3.16 / 2.47 / 2.25.  On the
+// 5M-sequence database (7% of the cells in pairs) the longest pair takes 21 ms
+// against 27 ms of throughput at query length 144 and the widths differ by
+// less than 5%: W stays.
 static int group_width_by_latency(const swh_ctx* c, const int* qlens, int nq, const swh_db* db,
                                   int W) {
     if (db->n_pairs == 0 || db->max_rows_pair == 0 || db->residues == 0) return W;
@@ -716,20 +764,16 @@ static int group_width_by_latency(const swh_ctx* c, const int* qlens, int nq, co
         cells += (uint64_t)qlens[i] * db->residues;
         qmax = std::max(qmax, qlens[i]);
     }
-    const double t_tp = (double)cells / (9.4e12 * std::max(c->num_cus, 1) / 256.0);
-    const double gf = (double)db->residues_pair / (double)db->residues;
+    const GroupLaunchModel m = group_model(c, db, cells);
     const int min_lanes = group_min_lanes(c, db->max_rows_pair, c->border_budget);
     auto predict = [&](int w) {
-        const double step = w == 8 ? 105.0 : w == 16 ? 159.0 : 238.0;
-        const double k = w == 8 ? 1.85 : w == 16 ? 1.55 : 1.45;
-        const int passes_max = group_pass_budget(c, cells, db->max_rows_pair, w);
+        const int budget = group_pass_budget(c, cells, db->max_rows_pair, w);
+        const int passes_max = group_passes(c, db, cells, qmax, w, budget, min_lanes);
         const int G = group_lanes_for(qmax, w, passes_max, min_lanes);
         const int strips = (qmax + w - 1) / w;
         const int passes = (strips + G - 1) / G;
-        const double use =
-            (double)strips / ((double)passes * G) * (double)((kWave / G) * G) / kWave;
-        const double crit = passes * ((double)db->max_rows_pair + G - 1) * step * 6.0 / 2.4e9;
-        return std::max(k * t_tp * (gf / use + (1.0 - gf)), crit);
+        return std::max(m.throughput_s(w, GroupLaunchModel::use(qmax, w, G)),
+                        passes * m.pass_s(w, G));
     };
     int best = W;
     double t_best = predict(W) * 0.95;
@@ -915,7 +959,10 @@ int enqueue_tasks(swh_ctx* c, swh_db* db, const Task* t, int nt, const Shape& sh
         QueryDesc& Gd = hd[nt + i];
         Gd = base;
         Gd.qpad = ((T.qlen + sh.Wg - 1) / sh.Wg) * sh.Wg;
-        Gd.group = group_lanes(c, T.qlen, sh.Wg, o, max_passes, min_lanes);
+        Gd.group = group_lanes(c, T.qlen, sh.Wg, o,
+                               group_passes(c, db, launch_cells, T.qlen, sh.Wg, max_passes,
+                                            min_lanes),
+                               min_lanes);
         Gd.blk_begin = T.pair_begin;
         Gd.blk_end = T.pair_end;
         const int per_item = kWavesPerWG * (kWave / Gd.group);

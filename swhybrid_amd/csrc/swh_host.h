@@ -376,6 +376,8 @@ void order_items(const swh_db* db, const Shape& sh, const QueryDesc* lane_d,
 // Group-mode shape helpers of the launch planning (swh_capi.cpp).
 int group_min_lanes(const swh_ctx* c, uint32_t rows_max, uint64_t border_budget);
 int group_pass_budget(const swh_ctx* c, uint64_t launch_cells, uint32_t rows_max, int W);
+int group_passes(const swh_ctx* c, const swh_db* db, uint64_t launch_cells, int qlen, int W,
+                 int max_passes, int min_lanes);
 int group_lanes(const swh_ctx* c, int qlen, int Wg, const swh_search_opts& o, int max_passes,
                 int min_lanes);
 // An int32 rescore launch and the strip-border bytes it needs.

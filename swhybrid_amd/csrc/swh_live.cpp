@@ -94,7 +94,9 @@ void LiveScan::release() {
 
 int LiveScan::group_lanes_of(int qlen) const {
     const uint64_t cells = (uint64_t)qlen * db_->residues;
-    const int passes = group_pass_budget(c_, cells, db_->max_rows_pair, sh_.Wg);
+    const int passes = group_passes(c_, db_, cells, qlen, sh_.Wg,
+                                    group_pass_budget(c_, cells, db_->max_rows_pair, sh_.Wg),
+                                    min_lanes_);
     int G = group_lanes(c_, qlen, sh_.Wg, o_, passes, min_lanes_);
     const int strips = (qlen + sh_.Wg - 1) / sh_.Wg;
     // a query swept in several passes keeps a border per group: no more
