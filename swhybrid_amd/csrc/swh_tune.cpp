@@ -57,6 +57,8 @@ struct DbHolder {
 // factor on the sample to be kept (64 batched queries on the 5M-sequence
 // database: fixed per-bucket choices from a 1/8 sample ran 1% slower).
 constexpr double kFixedGroupMargin = 0.98;
+// Configurations timed within this factor of the fastest are timed again.
+constexpr double kRetimeWithin = 1.2;
 
 extern "C" {
 
@@ -225,19 +227,38 @@ int swh_autotune(swh_ctx* c, const uint8_t* residues, const uint64_t* offsets,
         const int L = bucket_qlens[b];
         E.qlen_max = L;
         const int q0 = b * qpb;
-        double best = 1e300;
-        for (int k = 0; k < SWH_TUNE_LANE_CONFIGS; ++k) {
+        // A launch on the sample now and then runs 12-18% slow (never fast):
+        // every configuration within kRetimeWithin of the fastest is timed
+        // again and keeps its faster time, so one slow launch of the best
+        // configuration does not hand its bucket to another (over three
+        // tunings of the 5M-sequence database, one bucket in each had gone
+        // to a configuration 4-10% slower).
+        auto lane_opts = [](int k) {
             swh_search_opts o{};
             o.strip_width = kLaneCfgs[k].width;
             o.profile_mode = kLaneCfgs[k].pair ? SWH_PROFILE_PAIR : SWH_PROFILE_SINGLE;
-            double ms;
-            if ((rc = timed(h.db, q0, qpb, o, &ms))) return restore(rc);
+            return o;
+        };
+        double ms_k[SWH_TUNE_LANE_CONFIGS];
+        double fastest = 1e300;
+        for (int k = 0; k < SWH_TUNE_LANE_CONFIGS; ++k) {
+            if ((rc = timed(h.db, q0, qpb, lane_opts(k), &ms_k[k]))) return restore(rc);
+            fastest = std::min(fastest, ms_k[k]);
+        }
+        for (int k = 0; k < SWH_TUNE_LANE_CONFIGS; ++k) {
+            if (ms_k[k] > fastest * kRetimeWithin) continue;
+            double again;
+            if ((rc = timed(h.db, q0, qpb, lane_opts(k), &again))) return restore(rc);
+            ms_k[k] = std::min(ms_k[k], again);
+        }
+        double best = 1e300;
+        for (int k = 0; k < SWH_TUNE_LANE_CONFIGS; ++k) {
             const int w = kLaneCfgs[k].width;
-            E.lane_cost[k] = (float)(ms / ((double)qpb * lane_cols(L, w)));
-            if (ms < best) {
-                best = ms;
+            E.lane_cost[k] = (float)(ms_k[k] / ((double)qpb * lane_cols(L, w)));
+            if (ms_k[k] < best) {
+                best = ms_k[k];
                 E.strip_width = w;
-                E.profile_mode = o.profile_mode;
+                E.profile_mode = lane_opts(k).profile_mode;
             }
         }
         // group mode: the launch-time choice (width 0 and lanes 0 in the table)
@@ -247,12 +268,13 @@ int swh_autotune(swh_ctx* c, const uint8_t* residues, const uint64_t* offsets,
         E.group_lanes = 0;
         E.group_width = 0;
         if (!has_pairs) continue;
+        double base_ms;   // the launch-time group choice
         {
             swh_search_opts o{};
             o.strip_width = E.strip_width;
             o.profile_mode = E.profile_mode;
-            if ((rc = timed(h.db, q0, qpb, o, &best))) return restore(rc);
-            best *= kFixedGroupMargin;
+            if ((rc = timed(h.db, q0, qpb, o, &base_ms))) return restore(rc);
+            best = base_ms * kFixedGroupMargin;
         }
         std::vector<int> widths = {8, 16};
         if (E.strip_width == 24) widths.push_back(24);
@@ -276,6 +298,23 @@ int swh_autotune(swh_ctx* c, const uint8_t* residues, const uint64_t* offsets,
                     E.group_lanes = g;
                     E.group_width = wg;
                 }
+            }
+        }
+        if (E.group_width) {
+            // a fixed choice won: time both again (a slow launch of the
+            // launch-time choice must not be what made it win)
+            swh_search_opts o{};
+            o.strip_width = E.strip_width;
+            o.profile_mode = E.profile_mode;
+            double again;
+            if ((rc = timed(h.db, q0, qpb, o, &again))) return restore(rc);
+            base_ms = std::min(base_ms, again);
+            o.group_width = E.group_width;
+            o.group_lanes = E.group_lanes;
+            if ((rc = timed(h.db, q0, qpb, o, &again))) return restore(rc);
+            if (std::min(best, again) >= base_ms * kFixedGroupMargin) {
+                E.group_lanes = 0;
+                E.group_width = 0;
             }
         }
     }
