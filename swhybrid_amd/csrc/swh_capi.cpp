@@ -278,6 +278,31 @@ static int create_layout(swh_ctx* c, const uint8_t* residues, const uint64_t* of
     db->uid = next_uid.fetch_add(1);
     db->nseq = nseq;
     db->group_min_len = group_min;
+    // Residues go to the device as they are (the span the offsets cover, or
+    // a compact copy when that span is mostly gaps) and are interleaved there
+    // by k_pack: the host only plans the layout.  Page-locked ones (a worker's
+    // chunk) are queued first, so the copy engine carries them while the host
+    // plans (512 MB: 9.3 ms of copy beside about 10 ms of planning).
+    uint64_t total = 0, lo = UINT64_MAX, hi = 0;
+    for (uint64_t i = 0; i < nseq; ++i) {
+        total += lengths[i];
+        if (lengths[i] > 0) {
+            lo = std::min(lo, offsets[i]);
+            hi = std::max(hi, offsets[i] + lengths[i]);
+        }
+    }
+    if (lo > hi) lo = hi = 0;
+    const bool gappy = hi - lo > 2 * total + (uint64_t(64) << 20);
+    const bool direct = !gappy && hi - lo >= kMinEngineCopy && is_pinned_host(residues + lo);
+    std::lock_guard<std::mutex> stage_lock(c->stage_mu);
+    DevBuf& raw = c->stage_raw;
+    hipError_t e = hipSuccess;
+    if (direct) {
+        const uint64_t n = hi - lo;
+        e = grow_beside(c, raw, n + n / 4);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(raw.p, residues + lo, n, hipMemcpyHostToDevice, c->upload);
+    }
     // sort by length, longest first (ties keep caller order): the work queues
     // hand out the longest blocks first
     std::vector<uint32_t> order(nseq);
@@ -286,12 +311,8 @@ static int create_layout(swh_ctx* c, const uint8_t* residues, const uint64_t* of
     if (!std::is_sorted(lengths, lengths + nseq, std::greater<uint32_t>()))
         std::stable_sort(order.begin(), order.end(),
                          [&](uint32_t a, uint32_t b) { return lengths[a] > lengths[b]; });
-    uint64_t total = 0;
     uint64_t n_long = 0;
-    for (uint64_t i = 0; i < nseq; ++i) {
-        total += lengths[i];
-        if (lengths[i] >= group_min) ++n_long;
-    }
+    for (uint64_t i = 0; i < nseq; ++i) n_long += lengths[i] >= group_min;
     db->residues = total;
     const int np = (int)((n_long + 1) / 2);
     const uint64_t lane_first = std::min<uint64_t>(nseq, (uint64_t)np * 2);
@@ -346,21 +367,11 @@ static int create_layout(swh_ctx* c, const uint8_t* residues, const uint64_t* of
             }
         }
     });
-    // Residues go to the device as they are (the span the offsets cover, or
-    // a compact copy when that span is mostly gaps) and are interleaved there
-    // by k_pack: the host only plans the layout.
-    uint64_t lo = UINT64_MAX, hi = 0;
-    for (uint64_t i = 0; i < nseq; ++i)
-        if (lengths[i] > 0) {
-            lo = std::min(lo, offsets[i]);
-            hi = std::max(hi, offsets[i] + lengths[i]);
-        }
-    if (lo > hi) lo = hi = 0;
     std::vector<uint64_t> soff(std::max<uint64_t>(nseq, 1), 0);
     std::vector<uint8_t> compact;
     const uint8_t* stage = residues + lo;
     uint64_t stage_bytes = hi - lo;
-    if (stage_bytes > 2 * total + (uint64_t(64) << 20)) {
+    if (gappy) {
         compact.resize(total);
         uint64_t at = 0;
         for (uint64_t i = 0; i < nseq; ++i) {
@@ -381,22 +392,20 @@ static int create_layout(swh_ctx* c, const uint8_t* residues, const uint64_t* of
     // hipFree here (it would wait for every search running on the device).
     // Every table goes up from page-locked staging in one copy of at least
     // kMinEngineCopy bytes (padded), on the upload stream.
-    std::lock_guard<std::mutex> stage_lock(c->stage_mu);
     auto padded = [](size_t bytes) {
         return (std::max(bytes, kMinEngineCopy) + 4095) / 4096 * 4096;
     };
-    // residues: page-locked ones (a worker's chunk) go up as they are; small
-    // pageable ones through the staging; large pageable ones (a database
-    // built once) as a prefix the runtime stages itself, in multiples of
-    // kMinEngineCopy, and a padded tail through the staging
-    const bool direct = stage_bytes >= kMinEngineCopy && is_pinned_host(stage);
+    // other residues: small pageable ones through the staging; large
+    // pageable ones (a database built once) as a prefix the runtime stages
+    // itself, in multiples of kMinEngineCopy, and a padded tail through the
+    // staging
     const bool split = !direct && stage_bytes > (size_t(64) << 20);
     const size_t head = split ? stage_bytes / kMinEngineCopy * kMinEngineCopy : 0;
     const size_t tables[5] = {db->n_blocks * sizeof(BlockDesc), perm.size() * 4, slen.size() * 4,
                               sblk.size() * 4, soff.size() * 8};
     size_t h_need = direct ? 0 : padded(stage_bytes - head);
     for (size_t t : tables) h_need += padded(t);
-    hipError_t e = grow_beside(c, c->h_stage, h_need + h_need / 4);
+    if (e == hipSuccess) e = grow_beside(c, c->h_stage, h_need + h_need / 4);
     char* h_at = (char*)c->h_stage.p;
     // bytes at h into d at offset at, from the staging, padded
     auto up = [&](DevBuf& d, size_t at, const void* h, size_t bytes, bool cached) -> hipError_t {
@@ -408,7 +417,6 @@ static int create_layout(swh_ctx* c, const uint8_t* residues, const uint64_t* of
         h_at += n;
         return r;
     };
-    DevBuf& raw = c->stage_raw;
     DevBuf& d_soff = c->stage_soff;
     db->dev_bytes = woff * 4 + tables[0] + tables[1] + tables[2] + tables[3];
     if (e == hipSuccess) e = c->cache.take(db->words, std::max<uint64_t>(woff, 1) * 4);
@@ -417,11 +425,7 @@ static int create_layout(swh_ctx* c, const uint8_t* residues, const uint64_t* of
     if (e == hipSuccess) e = up(db->sorted_len, 0, slen.data(), tables[2], true);
     if (e == hipSuccess) e = up(db->sorted_blk, 0, sblk.data(), tables[3], true);
     if (e == hipSuccess) e = up(d_soff, 0, soff.data(), tables[4], false);
-    if (e == hipSuccess && direct) {
-        e = grow_beside(c, raw, stage_bytes + stage_bytes / 4);
-        if (e == hipSuccess)
-            e = hipMemcpyAsync(raw.p, stage, stage_bytes, hipMemcpyHostToDevice, c->upload);
-    } else if (e == hipSuccess) {
+    if (e == hipSuccess && !direct) {
         // sized once, as up() below sizes it: the prefix copy must land in the
         // buffer the tail goes to
         const size_t tail_n = padded(stage_bytes - head);
