@@ -734,11 +734,17 @@ static GroupLaunchModel group_model(const swh_ctx* c, const swh_db* db, uint64_t
 // Passes a query's group items may make: the launch's pass budget
 // (group_pass_budget), unless the best group within it leaves more than a
 // tenth of the wave's lanes idle; then as many as keep the longest pair's
-// sweep within the launch's predicted time.  On 100k sequences of up to 2,000
-// residues the budget allows one pass: 42 and 64 lanes a group at query
-// length 1000 and 2005 (7.0 and 14.1 ms), against 21 lanes in 2 and 4 passes
-// (4.4 and 8.9 ms).  On the 5M-sequence and the overflow-stress databases the
-// budget's groups use 94% of the lanes or more.
+// sweep within kSweepShare of the launch's predicted time.  On 100k sequences
+// of up to 2,000 residues the budget allows one pass: 42 and 64 lanes a group
+// at query length 1000 and 2005 (7.0 and 14.1 ms), against 21 lanes in 2 and
+// 4 passes (4.4 and 8.9 ms, the longest pair's sweep 55% of the launch).  A
+// worker's first chunk of the 5M-sequence database, its 117,771 longest
+// sequences, all in pairs of up to 34,988 rows, at query length 5478: 64
+// lanes in 4 passes; 21 lanes in 11, a sweep as long as the predicted launch,
+// made the pipeline's kernels 11% slower.  On the whole 5M-sequence and the
+// overflow-stress databases the budget's groups use 94% of the lanes or more.
+constexpr double kSweepShare = 0.6;
+
 int group_passes(const swh_ctx* c, const swh_db* db, uint64_t launch_cells, int qlen, int W,
                  int max_passes, int min_lanes) {
     const int G = group_lanes_for(qlen, W, max_passes, min_lanes);
@@ -746,7 +752,7 @@ int group_passes(const swh_ctx* c, const swh_db* db, uint64_t launch_cells, int 
     if (u >= 0.9 || db->max_rows_pair == 0) return max_passes;
     const GroupLaunchModel m = group_model(c, db, launch_cells);
     const double t_launch = m.throughput_s(W, 0.95);
-    const int p = (int)(t_launch / m.pass_s(W, kWave));
+    const int p = (int)(kSweepShare * t_launch / m.pass_s(W, kWave));
     return std::max(max_passes, std::min(kWave, p));
 }
 
