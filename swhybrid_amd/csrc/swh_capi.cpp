@@ -350,81 +350,101 @@ static int create_layout(swh_ctx* c, const uint8_t* residues, const uint64_t* of
         db->blk_real[b + 1] = db->blk_real[b] + real;
         db->blk_padded[b + 1] = db->blk_padded[b] + (uint64_t)bd.rows * n;
     }
-    const uint64_t nslots = lane_first + (uint64_t)nl * kLaneSeqs + 2;
-    std::vector<uint32_t> perm(nslots, kNoSeq);
-    std::vector<uint32_t> slen(nslots, 0);
-    std::vector<uint32_t> sblk(nslots, 0);
-    parallel_for(db->n_blocks, [&](int64_t b0, int64_t b1) {
-        for (int64_t b = b0; b < b1; ++b) {
-            const BlockDesc& bd = db->blocks[b];
-            const uint64_t n = b < np ? 2 : kLaneSeqs;
-            for (uint64_t s = bd.first; s < bd.first + n; ++s) {
-                sblk[s] = (uint32_t)b;
-                if (s < nseq) {
-                    perm[s] = order[s];
-                    slen[s] = lengths[order[s]];
-                }
-            }
-        }
-    });
-    std::vector<uint64_t> soff(std::max<uint64_t>(nseq, 1), 0);
-    std::vector<uint8_t> compact;
-    const uint8_t* stage = residues + lo;
-    uint64_t stage_bytes = hi - lo;
-    if (gappy) {
-        compact.resize(total);
-        uint64_t at = 0;
-        for (uint64_t i = 0; i < nseq; ++i) {
-            soff[i] = at;
-            at += lengths[i];
-        }
-        parallel_for((int64_t)nseq, [&](int64_t i0, int64_t i1) {
-            for (int64_t i = i0; i < i1; ++i)
-                std::memcpy(compact.data() + soff[i], residues + offsets[i], lengths[i]);
-        });
-        stage = compact.data();
-        stage_bytes = total;
-    } else {
-        for (uint64_t i = 0; i < nseq; ++i) soff[i] = lengths[i] > 0 ? offsets[i] - lo : 0;
-    }
-    // device buffers come from the context's spares where one fits and the
-    // staging buffers are the context's own and only grow, with slack: no
-    // hipFree here (it would wait for every search running on the device).
-    // Every table goes up from page-locked staging in one copy of at least
-    // kMinEngineCopy bytes (padded), on the upload stream.
+    // The tables are written straight into the page-locked staging, each at
+    // an offset padded to a copy of at least kMinEngineCopy bytes, and go up
+    // from there (no pageable copies to fill and copy again: 2.1M sequences of
+    // a 512 MB chunk took 20 ms that way).  The staging only grows, with
+    // slack; its old buffer is kept (grow_beside), not freed.
     auto padded = [](size_t bytes) {
         return (std::max(bytes, kMinEngineCopy) + 4095) / 4096 * 4096;
     };
+    const uint64_t nslots = lane_first + (uint64_t)nl * kLaneSeqs + 2;
+    std::vector<uint8_t> compact;   // the residues, when their span is mostly gaps
+    const uint8_t* stage = residues + lo;
+    uint64_t stage_bytes = hi - lo;
+    if (gappy) stage_bytes = total;
     // other residues: small pageable ones through the staging; large
     // pageable ones (a database built once) as a prefix the runtime stages
     // itself, in multiples of kMinEngineCopy, and a padded tail through the
     // staging
     const bool split = !direct && stage_bytes > (size_t(64) << 20);
     const size_t head = split ? stage_bytes / kMinEngineCopy * kMinEngineCopy : 0;
-    const size_t tables[5] = {db->n_blocks * sizeof(BlockDesc), perm.size() * 4, slen.size() * 4,
-                              sblk.size() * 4, soff.size() * 8};
-    size_t h_need = direct ? 0 : padded(stage_bytes - head);
-    for (size_t t : tables) h_need += padded(t);
+    const size_t tables[5] = {db->n_blocks * sizeof(BlockDesc), nslots * 4, nslots * 4,
+                              nslots * 4, std::max<uint64_t>(nseq, 1) * 8};
+    size_t region[6];   // offsets in the staging: blocks, perm, slen, sblk, soff, residue tail
+    size_t h_need = 0;
+    for (int t = 0; t < 5; ++t) {
+        region[t] = h_need;
+        h_need += padded(tables[t]);
+    }
+    region[5] = h_need;
+    if (!direct) h_need += padded(stage_bytes - head);
     if (e == hipSuccess) e = grow_beside(c, c->h_stage, h_need + h_need / 4);
-    char* h_at = (char*)c->h_stage.p;
-    // bytes at h into d at offset at, from the staging, padded
-    auto up = [&](DevBuf& d, size_t at, const void* h, size_t bytes, bool cached) -> hipError_t {
+    char* const hs = (char*)c->h_stage.p;
+    if (e == hipSuccess) {
+        if (tables[0]) std::memcpy(hs + region[0], db->blocks.data(), tables[0]);
+        uint32_t* perm = (uint32_t*)(hs + region[1]);
+        uint32_t* slen = (uint32_t*)(hs + region[2]);
+        uint32_t* sblk = (uint32_t*)(hs + region[3]);
+        uint64_t* soff = (uint64_t*)(hs + region[4]);
+        // slots past every block (the two spare ones at the end)
+        const uint64_t covered = std::max<uint64_t>((uint64_t)np * 2,
+                                                    lane_first + (uint64_t)nl * kLaneSeqs);
+        for (uint64_t s2 = covered; s2 < nslots; ++s2) {
+            perm[s2] = kNoSeq;
+            slen[s2] = 0;
+            sblk[s2] = 0;
+        }
+        parallel_for(db->n_blocks, [&](int64_t b0, int64_t b1) {
+            for (int64_t b = b0; b < b1; ++b) {
+                const BlockDesc& bd = db->blocks[b];
+                const uint64_t n = b < np ? 2 : kLaneSeqs;
+                for (uint64_t s2 = bd.first; s2 < bd.first + n; ++s2) {
+                    sblk[s2] = (uint32_t)b;
+                    const bool real = s2 < nseq;
+                    perm[s2] = real ? order[s2] : kNoSeq;
+                    slen[s2] = real ? lengths[order[s2]] : 0;
+                }
+            }
+        });
+        soff[0] = 0;
+        if (gappy) {
+            uint64_t at = 0;
+            for (uint64_t i = 0; i < nseq; ++i) {
+                soff[i] = at;
+                at += lengths[i];
+            }
+            compact.resize(total);
+            parallel_for((int64_t)nseq, [&](int64_t i0, int64_t i1) {
+                for (int64_t i = i0; i < i1; ++i)
+                    std::memcpy(compact.data() + soff[i], residues + offsets[i], lengths[i]);
+            });
+            stage = compact.data();
+        } else {
+            parallel_for((int64_t)nseq, [&](int64_t i0, int64_t i1) {
+                for (int64_t i = i0; i < i1; ++i)
+                    soff[i] = lengths[i] > 0 ? offsets[i] - lo : 0;
+            });
+        }
+        if (!direct && stage_bytes > head)
+            std::memcpy(hs + region[5], stage + head, stage_bytes - head);
+    }
+    // region t of the staging into d at offset at, padded
+    auto up = [&](DevBuf& d, size_t at, int t, size_t bytes, bool cached) -> hipError_t {
         const size_t n = padded(bytes);
         hipError_t r = cached ? c->cache.take(d, at + n) : grow_beside(c, d, at + n + n / 4);
         if (r != hipSuccess) return r;
-        if (bytes) std::memcpy(h_at, h, bytes);
-        r = hipMemcpyAsync((char*)d.p + at, h_at, n, hipMemcpyHostToDevice, c->upload);
-        h_at += n;
-        return r;
+        return hipMemcpyAsync((char*)d.p + at, hs + region[t], n, hipMemcpyHostToDevice,
+                              c->upload);
     };
     DevBuf& d_soff = c->stage_soff;
     db->dev_bytes = woff * 4 + tables[0] + tables[1] + tables[2] + tables[3];
     if (e == hipSuccess) e = c->cache.take(db->words, std::max<uint64_t>(woff, 1) * 4);
-    if (e == hipSuccess) e = up(db->d_blocks, 0, db->blocks.data(), tables[0], true);
-    if (e == hipSuccess) e = up(db->perm, 0, perm.data(), tables[1], true);
-    if (e == hipSuccess) e = up(db->sorted_len, 0, slen.data(), tables[2], true);
-    if (e == hipSuccess) e = up(db->sorted_blk, 0, sblk.data(), tables[3], true);
-    if (e == hipSuccess) e = up(d_soff, 0, soff.data(), tables[4], false);
+    if (e == hipSuccess) e = up(db->d_blocks, 0, 0, tables[0], true);
+    if (e == hipSuccess) e = up(db->perm, 0, 1, tables[1], true);
+    if (e == hipSuccess) e = up(db->sorted_len, 0, 2, tables[2], true);
+    if (e == hipSuccess) e = up(db->sorted_blk, 0, 3, tables[3], true);
+    if (e == hipSuccess) e = up(d_soff, 0, 4, tables[4], false);
     if (e == hipSuccess && !direct) {
         // sized once, as up() below sizes it: the prefix copy must land in the
         // buffer the tail goes to
@@ -432,7 +452,7 @@ static int create_layout(swh_ctx* c, const uint8_t* residues, const uint64_t* of
         e = grow_beside(c, raw, head + tail_n + tail_n / 4);
         if (e == hipSuccess && head)
             e = hipMemcpyAsync(raw.p, stage, head, hipMemcpyHostToDevice, c->upload);
-        if (e == hipSuccess) e = up(raw, head, stage + head, stage_bytes - head, false);
+        if (e == hipSuccess) e = up(raw, head, 5, stage_bytes - head, false);
     }
     if (e == hipSuccess && db->n_blocks > 0)
         e = launch_pack((const BlockDesc*)db->d_blocks.p, db->n_blocks, np,

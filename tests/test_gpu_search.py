@@ -399,6 +399,25 @@ def test_short_query_over_long_sequences_default_shape(eng, qlen):
         d.close()
 
 
+def test_sequences_scattered_over_a_sparse_buffer(eng):
+    """Sequences whose span in the caller's buffer is mostly gaps (here 80 MB
+    for 0.6 MB of residues) go up as a compact copy; scores equal the oracle's."""
+    rng = np.random.Generator(np.random.PCG64(105))
+    dense = small_db(2000, 106)
+    buf = np.full(80 << 20, 23, dtype=np.uint8)
+    starts = np.sort(rng.choice((80 << 20) // 4096 - 1, dense.n, replace=False)) * 4096
+    for i in range(dense.n):
+        buf[starts[i]:starts[i] + dense.lengths[i]] = dense.seq(i)
+    sparse = G.SeqDB(buf, starts.astype(np.uint64), dense.lengths.copy())
+    q = G.queries([300], 107)[0]
+    d = eng.load_db(sparse)
+    try:
+        assert np.array_equal(eng.search(d, q)[0],
+                              O.search_i32(q, dense, eng.matrix, eng.gap_open, eng.gap_extend))
+    finally:
+        d.close()
+
+
 def test_context_destroy_refused_while_its_databases_exist():
     """swh_ctx_destroy refuses while a database made on the context exists
     (its buffers go back to the context when it is destroyed); Engine.close
