@@ -333,8 +333,11 @@ def hybrid_search(db: SeqDB, query, matrix="blosum62", gap_open=10, gap_extend=2
                   hip_workers=1, cpu_threads=0, chunk_bytes=0, cpu_chunk_bytes=0,
                   all_scores=False, first_chunk_bytes=0):
     """The worker pipeline with HIPWorkers and a CPU worker (AVX2, cpu_threads
-    threads) pulling chunks from one DataPool.  Returns dict(top_score,
-    top_index, scores, report)."""
+    threads) pulling chunks from one DataPool.  chunk_bytes / first_chunk_bytes:
+    a HIPWorker's chunk schedule (its first chunk, each later one twice the one
+    before, up to chunk_bytes; both 0 = the default schedule by query length).
+    Returns dict(top_score, top_index, scores, report); report names the
+    schedule that ran."""
     m = _as_matrix(matrix)
     q = _as_query(query)
     res = np.ascontiguousarray(db.residues, dtype=np.uint8)
