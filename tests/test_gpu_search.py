@@ -90,6 +90,31 @@ def test_single_sequence_and_empty(eng):
     assert got.shape == (0,) and st["cells"] == 0
 
 
+def test_zero_length_sequences_among_others(eng):
+    """Zero-length sequences anywhere in the database (first, between others,
+    last) score 0; the rest score as the oracle's, with an odd number of long
+    sequences (a pair block with one empty slot)."""
+    rng = np.random.Generator(np.random.PCG64(108))
+    seqs = [np.zeros(0, np.uint8)]
+    for k in range(301):
+        seqs.append(G.random_residues(rng, int(rng.integers(20, 400))))
+        if k % 50 == 0:
+            seqs.append(np.zeros(0, np.uint8))
+    seqs += [G.random_residues(rng, 2100 + 7 * k) for k in range(3)]   # three long ones
+    seqs.append(np.zeros(0, np.uint8))
+    db = G.SeqDB.from_list(seqs)
+    q = G.queries([150], 109)[0]
+    for gmin in (0, 2048):
+        d = eng.load_db(db, group_min_len=gmin)
+        try:
+            got, _ = eng.search(d, q)
+            want = O.search_i32(q, db, eng.matrix, eng.gap_open, eng.gap_extend)
+            assert np.array_equal(got, want)
+            assert (got[db.lengths == 0] == 0).all()
+        finally:
+            d.close()
+
+
 def test_pad_and_unknown_codes(eng):
     rng = np.random.Generator(np.random.PCG64(8))
     seqs = [G.random_residues(rng, 120) for _ in range(130)]
