@@ -722,8 +722,8 @@ static double builtin_cost(int k) {
 // instructions with wait states at W = 8, 16, 24, at about 6 cycles an
 // instruction for a wave that shares its SIMD.  Throughput: the launch's
 // cells at 9.4 T cells/s (per 256 CUs), the group share of them divided by
-// the lanes' use, times what a group-heavy launch reaches of that at each
-// width (1.45, 1.55, 1.85 at W = 24, 16, 8).
+// the lanes' use and by what group mode reaches of that rate at each width
+// (1 / 1.45, 1 / 1.55, 1 / 1.85 at W = 24, 16, 8).
 struct GroupLaunchModel {
     double t_tp = 0;   // the launch's cells at the scan's full rate, seconds
     double gf = 0;     // share of the cells in group pairs
@@ -739,7 +739,7 @@ struct GroupLaunchModel {
         return ((double)rows_max + G - 1) * step(w) * 6.0 / 2.4e9;
     }
     double throughput_s(int w, double lane_use) const {
-        return k(w) * t_tp * (gf / lane_use + (1.0 - gf));
+        return t_tp * (k(w) * gf / lane_use + (1.0 - gf));
     }
 };
 
@@ -782,9 +782,10 @@ int group_passes(const swh_ctx* c, const swh_db* db, uint64_t launch_cells, int 
 // residues, 90% of the cells in group pairs (ms at W = 8 / 16 / 24): query 50
 // 0.53 / 0.81 / 1.05, 100 0.74 / 0.81 / 1.07, 144 0.95 / 0.82 / 1.11, 200
 // 1.42 / 1.19 / 1.15, 256 1.47 / 1.22 / 1.37, 464 3.16 / 2.47 / 2.25.  On the
-// 5M-sequence database (7% of the cells in pairs) the longest pair takes 21 ms
-// against 27 ms of throughput at query length 144 and the widths differ by
-// less than 5%: W stays.
+// 5M-sequence database (7% of the cells in pairs, pairs of up to 34,988 rows)
+// the widths' times differ by less than 5% from query length 144 up, and
+// W = 24 stays from 200 up, where its 21 ms sweep is within kSweepShare of
+// the launch.
 static int group_width_by_latency(const swh_ctx* c, const int* qlens, int nq, const swh_db* db,
                                   int W) {
     if (db->n_pairs == 0 || db->max_rows_pair == 0 || db->residues == 0) return W;
@@ -796,26 +797,51 @@ static int group_width_by_latency(const swh_ctx* c, const int* qlens, int nq, co
     }
     const GroupLaunchModel m = group_model(c, db, cells);
     const int min_lanes = group_min_lanes(c, db->max_rows_pair, c->border_budget);
+    struct Est {
+        double tp, sweep;
+        double t() const { return std::max(tp, sweep); }
+        bool risky() const { return sweep > kSweepShare * tp; }
+    };
     auto predict = [&](int w) {
         const int budget = group_pass_budget(c, cells, db->max_rows_pair, w);
         const int passes_max = group_passes(c, db, cells, qmax, w, budget, min_lanes);
         const int G = group_lanes_for(qmax, w, passes_max, min_lanes);
         const int strips = (qmax + w - 1) / w;
         const int passes = (strips + G - 1) / G;
-        return std::max(m.throughput_s(w, GroupLaunchModel::use(qmax, w, G)),
-                        passes * m.pass_s(w, G));
+        return Est{m.throughput_s(w, GroupLaunchModel::use(qmax, w, G)), passes * m.pass_s(w, G)};
     };
-    int best = W;
-    double t_best = predict(W) * 0.95;
-    for (int w : {16, 8}) {
-        if (w == W) continue;
-        const double t = predict(w);
-        if (t < t_best) {
-            best = w;
-            t_best = t;
+    const int widths[3] = {24, 16, 8};
+    Est e[3];
+    int bi = -1;
+    for (int i = 0; i < 3; ++i) {
+        e[i] = predict(widths[i]);
+        if (widths[i] == W) bi = i;
+    }
+    if (bi < 0) return W;
+    double t_best = e[bi].t() * 0.95;
+    for (int i = 0; i < 3; ++i) {
+        if (i == bi || widths[i] == 24) continue;   // 24 only ever stays, never replaces
+        if (e[i].t() < t_best) {
+            bi = i;
+            t_best = e[i].t();
         }
     }
-    return best;
+    // A sweep above kSweepShare of the throughput time leaves the launch's
+    // end to one wave and its SIMD neighbours: at query length 144 on the
+    // 5M-sequence database (a 21 ms sweep at W = 24 in a 27 ms launch) the
+    // launch took 26.7-35 ms at W = 24, a steady 26.7 ms at W = 16.  A width
+    // whose sweep fits the share replaces it when predicted within 5%.
+    if (e[bi].risky()) {
+        const double t_cap = e[bi].t() * 1.05;
+        int alt = -1;
+        for (int i = 0; i < 3; ++i) {
+            if (i == bi || e[i].risky() || e[i].t() > t_cap) continue;
+            if (widths[i] == 24 && W != 24) continue;
+            if (alt < 0 || e[i].t() < e[alt].t()) alt = i;
+        }
+        if (alt >= 0) bi = alt;
+    }
+    return widths[bi];
 }
 
 // Launch shape shared by the queries of a launch.  Lane scan: the width and
