@@ -149,6 +149,20 @@ __device__ __forceinline__ void load_profile(RawProfile<W>& rp, const short* lds
     }
 }
 
+// The same rows from a lane's first staged column (col0) and the row stride in
+// bytes: one multiply-add per residue forms the LDS address.
+template <int W>
+__device__ __forceinline__ void load_profile_at(RawProfile<W>& rp, const char* col0,
+                                                uint32_t row_bytes, uint32_t a, uint32_t b) {
+    const uint4* pa = reinterpret_cast<const uint4*>(col0 + __umul24(a, row_bytes));
+    const uint4* pb = reinterpret_cast<const uint4*>(col0 + __umul24(b, row_bytes));
+#pragma unroll
+    for (int k = 0; k < W / 8; ++k) {
+        rp.va[k] = pa[k];
+        rp.vb[k] = pb[k];
+    }
+}
+
 template <int W>
 __device__ __forceinline__ void interleave_profile(s2 (&P)[W], const RawProfile<W>& rp) {
 #pragma unroll
@@ -452,9 +466,11 @@ __device__ __forceinline__ void lane_item(const ScanArgs& A, const QueryDesc& Qd
 // later through a wave-wide DPP shift (wave_shr:1), so the pair is swept as a
 // pipeline along the database with NR rows of skew per lane.  The
 // first lane of a group reads the previous pass's right edge from the border
-// array, the last active lane writes it for the next pass.
+// array, the last active lane writes it for the next pass.  Lane 0 of the
+// wave reads 0 from the shift (bound_ctrl), a value no lane uses: the first
+// lane of every group takes the border instead, so no old value is set up.
 __device__ __forceinline__ uint32_t wave_shr1(uint32_t v) {
-    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x138, 0xf, 0xf, false);
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x138, 0xf, 0xf, true);
 }
 
 template <int WL, int WG>
@@ -532,13 +548,19 @@ __device__ __forceinline__ void group_item(const ScanArgs& A, const QueryDesc& Q
             }
         }
         // a step's first profile row is loaded while the step before it
-        // computes, its other row at the start of the step
+        // computes, its other row at the start of the step.  Steps run in
+        // units of U starting at multiples of U (NR x U = 4 rows), so row
+        // r0 + k of step t0 + u sits at byte (NR u + k - NR g) & 3 of its word:
+        // four shifts per lane, fixed for the pass.
+        static_assert(NR * U == 4, "a unit of steps covers one residue word");
+        const char* col0 = reinterpret_cast<const char*>(lds + jj0);
+        const uint32_t row_bytes = (uint32_t)A.ps_group * sizeof(short);
+        uint32_t shv[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) shv[k] = 8u * (uint32_t)((k - NR * gl) & 3);
         RawProfile<W> rp;
-        {
-            const int sh = 8 * ((-NR * gl) & 3);
-            load_profile<W>(rp, lds, A.ps_group, (wa[0].x >> sh) & 0xff, (wa[0].y >> sh) & 0xff,
-                            jj0);
-        }
+        load_profile_at<W>(rp, col0, row_bytes, (wa[0].x >> shv[0]) & 0xff,
+                           (wa[0].y >> shv[0]) & 0xff);
         // U steps from (wc, bc) while the next U steps' words and borders load
         // into (wn, bn); unrolled twice with the two sets swapping roles
         auto unit = [&](int t0, const uint2 (&wc)[U], const uint2 (&bc)[U][NR], uint2 (&wn)[U],
@@ -557,14 +579,13 @@ __device__ __forceinline__ void group_item(const ScanArgs& A, const QueryDesc& Q
                 const RawProfile<W> cur = rp;
                 std::conditional_t<NR == 2, RawProfile<W>, char> rq;   // the second row
                 if constexpr (NR == 2) {
-                    const int sh = 8 * ((r0 + 1) & 3);
-                    load_profile<W>(rq, lds, A.ps_group, (wc[u].x >> sh) & 0xff,
-                                    (wc[u].y >> sh) & 0xff, jj0);
+                    const uint32_t sh = shv[(NR * u + 1) & 3];
+                    load_profile_at<W>(rq, col0, row_bytes, (wc[u].x >> sh) & 0xff,
+                                       (wc[u].y >> sh) & 0xff);
                 }
                 const uint2 nw = u + 1 < U ? wc[u + 1] : wn[0];
-                const int sh = 8 * ((r0 + NR) & 3);
-                load_profile<W>(rp, lds, A.ps_group, (nw.x >> sh) & 0xff, (nw.y >> sh) & 0xff,
-                                jj0);
+                const uint32_t sh = shv[(NR * u + NR) & 3];
+                load_profile_at<W>(rp, col0, row_bytes, (nw.x >> sh) & 0xff, (nw.y >> sh) & 0xff);
                 __builtin_amdgcn_sched_barrier(0);   // issue the loads before the step's work
                 // every lane takes part in the shifts (no lane may be masked off here)
                 uint32_t hl[NR], fl[NR];
@@ -575,7 +596,7 @@ __device__ __forceinline__ void group_item(const ScanArgs& A, const QueryDesc& Q
                     hl[j] = gl == 0 ? bc[u][j].x : hx;
                     fl[j] = gl == 0 ? bc[u][j].y : fx;
                 }
-                if (on && r0 >= 0 && r0 < rows) {
+                if (on && (uint32_t)r0 < (uint32_t)rows) {   // 0 <= r0 < rows
                     auto row = [&](const RawProfile<W>& raw, int j) {
                         s2 P[W];
                         interleave_profile<W>(P, raw);
