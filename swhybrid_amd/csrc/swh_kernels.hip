@@ -527,6 +527,10 @@ __device__ __forceinline__ void group_item(const ScanArgs& A, const QueryDesc& Q
         uint32_t ch[NR], cf[NR];   // this lane's last (H, F) of each row of its step
 #pragma unroll
         for (int j = 0; j < NR; ++j) ch[j] = cf[j] = kZero16x2;
+        // the last lane of a group writes its edge for the next pass; pend: the
+        // step before ran rows whose edge is not written yet
+        const bool edge_out = has_out && gl == nl - 1;
+        bool pend = false;
         constexpr int U = 4 / NR;   // steps per 4 rows
         const int steps = rmax > 0 ? (int)(rmax / NR) + nl - 1 : 0;
         // residue words and border entries are loaded 4 rows ahead; rows
@@ -596,7 +600,15 @@ __device__ __forceinline__ void group_item(const ScanArgs& A, const QueryDesc& Q
                     hl[j] = gl == 0 ? bc[u][j].x : hx;
                     fl[j] = gl == 0 ? bc[u][j].y : fx;
                 }
-                if (on && (uint32_t)r0 < (uint32_t)rows) {   // 0 <= r0 < rows
+                // the last lane's edge of the step before goes to the border
+                // array here, ahead of this step's rows, so that the rows, their
+                // E updates and maxima stay one block for the scheduler
+                if (pend) {
+#pragma unroll
+                    for (int j = 0; j < NR; ++j) bord[r0 - NR + j] = make_uint2(ch[j], cf[j]);
+                }
+                const bool run = on && (uint32_t)r0 < (uint32_t)rows;   // 0 <= r0 < rows
+                if (run) {
                     auto row = [&](const RawProfile<W>& raw, int j) {
                         s2 P[W];
                         interleave_profile<W>(P, raw);
@@ -609,18 +621,23 @@ __device__ __forceinline__ void group_item(const ScanArgs& A, const QueryDesc& Q
                     };
                     row(cur, 0);
                     if constexpr (NR == 2) row(rq, 1);
-                    if (has_out && gl == nl - 1) {
-#pragma unroll
-                        for (int j = 0; j < NR; ++j) bord[r0 + j] = make_uint2(ch[j], cf[j]);
-                    }
                 }
+                pend = run && edge_out;
                 __builtin_amdgcn_sched_barrier(0);
             }
         };
+        int t_end = 0;   // steps run
         for (int t0 = 0; t0 < steps; t0 += 2 * U) {
             unit(t0, wa, ba, wb, bb);
+            t_end = t0 + U;
             if (t0 + U >= steps) break;   // a uniform branch: no tail copy of the code
             unit(t0 + U, wb, bb, wa, ba);
+            t_end = t0 + 2 * U;
+        }
+        if (pend) {   // the last step's edge
+            const int r = NR * (t_end - 1 - gl);
+#pragma unroll
+            for (int j = 0; j < NR; ++j) bord[r + j] = make_uint2(ch[j], cf[j]);
         }
     }
     s_red[wave][lane] = bits(S);
