@@ -784,8 +784,8 @@ int group_passes(const swh_ctx* c, const swh_db* db, uint64_t launch_cells, int 
 // 1.42 / 1.19 / 1.15, 256 1.47 / 1.22 / 1.37, 464 3.16 / 2.47 / 2.25.  On the
 // 5M-sequence database (7% of the cells in pairs, pairs of up to 34,988 rows)
 // the widths' times differ by less than 5% from query length 144 up, and
-// W = 24 stays from 200 up, where its 21 ms sweep is within kSweepShare of
-// the launch.
+// from 200 up the width chosen by lane use stays: a 21 ms sweep at W = 24 is
+// within kSweepShare of those launches.
 static int group_width_by_latency(const swh_ctx* c, const int* qlens, int nq, const swh_db* db,
                                   int W) {
     if (db->n_pairs == 0 || db->max_rows_pair == 0 || db->residues == 0) return W;
