@@ -625,6 +625,11 @@ def main():
     # instruction per SIMD every 4 clocks at 2.4 GHz.
     secs = kernel_ms / 1e3
     traffic, insts_per_pair = measured_counters()
+    # the committed counters were read on the full workload; a run on another
+    # database or a subset of the queries reports no figures derived from them
+    full = db.n == 5_000_000 and not args.query_lengths
+    if not full:
+        traffic = insts_per_pair = None
     ops9 = OPS_PER_CELL * cells / secs / 1e12 if secs > 0 else 0.0
     tips = cells / 2 * insts_per_pair / secs / 1e12 if (insts_per_pair and secs > 0) else None
     useful = cells / 2 * MIN_INSTS_PER_CELL_PAIR / secs / 1e12 if secs > 0 else 0.0
@@ -669,8 +674,10 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "int16x2 saturating scan + int32 rescore",
         "data": "synthetic (seeded generator, Robinson-Robinson residue frequencies)",
-        "config": {"workload": "plain search, 5M sequences (~1.75G residues), query lengths "
-                               "144/464/1000/2005/5478, BLOSUM62 10/2, top-20",
+        "config": {"workload": (f"plain search, {db.n:,} sequences ({db.total / 1e9:.2f}G "
+                                f"residues), query lengths "
+                                f"{'/'.join(str(len(q)) for q in queries)}, BLOSUM62 10/2, "
+                                f"top-20"),
                    "nseq": db.n, "residues": db.total, "query_lengths": [len(q) for q in queries]},
         "roofline": roofline,
         "per_query": pq_out,
