@@ -1,8 +1,8 @@
 """HIPWorker drops into the reference's own tree: its two files compile against
 the reference's Worker.h (src/Worker.h:82-249) with the reference's compiler
 flags (Makefiles/Makefile.cuda:19,38), and every Worker symbol the object
-needs is defined by the reference's own Worker.cpp.  CPU only; skipped where
-the reference checkout is absent (the GPU box)."""
+needs is defined by the reference's own Worker.cpp.  CPU only; skipped on a
+machine without the reference checkout."""
 import os
 import re
 import shutil
