@@ -718,9 +718,11 @@ static double builtin_cost(int k) {
 
 // Time model of a group-mode launch, shared by the group width and pass
 // choices (group_width_by_latency).  A pair is swept by one wave, P passes of
-// rows + G - 1 steps; a step of W columns costs about 105, 159 and 238 VALU
-// instructions with wait states at W = 8, 16, 24, at about 6 cycles an
-// instruction for a wave that shares its SIMD.  Throughput: the launch's
+// rows + G - 1 steps; a step of W columns is priced at 105, 159 and 238 VALU
+// instructions at W = 8, 16, 24 and about 6 cycles an instruction for a wave
+// that shares its SIMD (the compiled rows with their wait states when the
+// model was fit to measured launches; the rows now compile to about 7% fewer
+// issue slots at W = 16 and 24, and the constants stay with the fit).  Throughput: the launch's
 // cells at 9.4 T cells/s (per 256 CUs), the group share of them divided by
 // the lanes' use and by what group mode reaches of that rate at each width
 // (1 / 1.45, 1 / 1.55, 1 / 1.85 at W = 24, 16, 8).
