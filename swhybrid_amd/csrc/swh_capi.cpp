@@ -721,7 +721,7 @@ static double builtin_cost(int k) {
 // rows + G - 1 steps; a step of W columns is priced at 105, 159 and 238 VALU
 // instructions at W = 8, 16, 24 and about 6 cycles an instruction for a wave
 // that shares its SIMD (the compiled rows with their wait states when the
-// model was fit to measured launches; the rows now compile to about 7% fewer
+// model was fit to measured launches; the rows now compile to 6% and 9% fewer
 // issue slots at W = 16 and 24, and the constants stay with the fit).  Throughput: the launch's
 // cells at 9.4 T cells/s (per 256 CUs), the group share of them divided by
 // the lanes' use and by what group mode reaches of that rate at each width
