@@ -722,10 +722,10 @@ static double builtin_cost(int k) {
 // instructions at W = 8, 16, 24 and about 6 cycles an instruction for a wave
 // that shares its SIMD (the compiled rows with their wait states when the
 // model was fit to measured launches; the rows now compile to 6% and 9% fewer
-// issue slots at W = 16 and 24, and the constants stay with the fit).  Throughput: the launch's
-// cells at 9.4 T cells/s (per 256 CUs), the group share of them divided by
-// the lanes' use and by what group mode reaches of that rate at each width
-// (1 / 1.45, 1 / 1.55, 1 / 1.85 at W = 24, 16, 8).
+// issue slots at W = 16 and 24, and the constants stay with the fit).
+// Throughput: the launch's cells at 9.4 T cells/s (per 256 CUs), the group
+// share of them divided by the lanes' use and by what group mode reaches of
+// that rate at each width (1 / 1.45, 1 / 1.55, 1 / 1.85 at W = 24, 16, 8).
 struct GroupLaunchModel {
     double t_tp = 0;   // the launch's cells at the scan's full rate, seconds
     double gf = 0;     // share of the cells in group pairs
